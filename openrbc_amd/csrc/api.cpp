@@ -1,0 +1,914 @@
+// api.cpp -- orbc_ctx and the C ABI of liborbc_hip.so (include/orbc.h).
+//
+// The context owns one particle store on the device: both reference
+// containers (LipidContainer, ProteContainer; src/container.h:117-157) merged
+// into one array kept in Morton order of the binning grid, FP64 state x,v,n,o
+// (x,n double-buffered for the fused step), FP32 mirrors for the mixed force
+// path, the Verlet list, and per-particle bond lists.  The host keeps each
+// species' upload order so results come back in the caller's order.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace orbc;
+
+static const double kRadius[NT] = {0.56125, 1.12375, 1.12375, 0.56125, 1.5, 0.5};  // src/forcefield_canonical.h:41
+
+namespace orbc {
+
+int fail(orbc_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+}  // namespace orbc
+
+#define guarded orbc_guarded
+
+namespace {
+
+int ceil_log2(long long v) {
+    int b = 0;
+    while ((1LL << b) < v) ++b;
+    return b;
+}
+
+void langevin_coeffs(const orbc_config &cfg, double *gamma, double *sigma) {
+    // src/integrate_langevin.h:262-266
+    for (int k = 0; k < NT; ++k) {
+        gamma[k] = 6.0 * M_PI * cfg.eta * kRadius[k];
+        sigma[k] = std::sqrt(2.0 * cfg.kBT * gamma[k]) * std::sqrt(3.0 / cfg.dt);
+    }
+}
+
+}  // namespace
+
+namespace orbc {
+
+void sync(orbc_ctx *c) { ORBC_HIP_CHECK(hipStreamSynchronize(c->stream)); }
+
+unsigned read_flag(orbc_ctx *c, int k) {
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->pinned + k, c->flags + k, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return c->pinned[k];
+}
+
+void zero_flag(orbc_ctx *c, int k) { launch_fill_u32(c->flags + k, 0u, 1, c->stream); }
+
+}  // namespace orbc
+
+namespace {
+
+ForceArgs force_args(orbc_ctx *c) {
+    ForceArgs a{};
+    a.n = c->n; a.stride = c->stride;
+    a.x = c->x[c->cur]; a.nrm = c->nrm[c->cur];
+    a.xq = c->xq[c->cur]; a.nq = c->nq[c->cur];
+    a.type = c->type; a.id = c->id;
+    a.nbr = c->nbr; a.nnbr = c->nnbr;
+    a.bnd = c->nbonds ? c->bnd : nullptr;
+    a.nbnd = c->nbonds ? c->nbnd : nullptr;
+    a.anc_grid = c->ag;
+    a.f32 = c->f32; a.t32 = c->t32; a.f64 = c->f64; a.t64 = c->t64;
+    a.e = c->e; a.w = c->w;
+    a.swizzle = 1;
+    return a;
+}
+
+IntegArgs integ_args(orbc_ctx *c) {
+    IntegArgs a{};
+    a.n = c->n;
+    a.x = c->x[c->cur]; a.v = c->v; a.nrm = c->nrm[c->cur]; a.o = c->o;
+    a.f32 = c->f32; a.t32 = c->t32; a.f64 = c->f64; a.t64 = c->t64;
+    a.type = c->type; a.id = c->id; a.anc = c->anc;
+    a.xq = c->mixed() ? c->xq[c->cur] : nullptr;
+    a.nq = c->mixed() ? c->nq[c->cur] : nullptr;
+    a.anc_grid = c->ag;
+    a.dt = c->cfg.dt;
+    langevin_coeffs(c->cfg, a.gamma, a.sigma);
+    a.seed = c->cfg.seed;
+    a.step = (unsigned long long)c->step;
+    for (int d = 0; d < 3; ++d) { a.box_lo[d] = c->cfg.box_lo[d]; a.box_hi[d] = c->cfg.box_hi[d]; }
+    a.zeta = c->scal;
+    a.ke_partial = c->red;
+    a.dr_opt = 5e-2; a.dn_opt = 5e-2;
+    return a;
+}
+
+double nh_Q(orbc_ctx *c) { return c->cfg.Q > 0 ? c->cfg.Q : 0.01 * (double)c->n; }  // set_Q, src/runtime_parameter.h:76
+
+// ------------------------------------------------------------------ commit
+void alloc_store(orbc_ctx *c) {
+    const long long n = c->n;
+    DevArena &m = c->mem;
+    for (int b = 0; b < 2; ++b) {
+        c->x[b] = m.alloc<double>(3 * n);
+        c->nrm[b] = m.alloc<double>(3 * n);
+        c->xq[b] = m.alloc<float4>(n);
+        c->nq[b] = m.alloc<float4>(n);
+    }
+    c->v = m.alloc<double>(3 * n); c->o = m.alloc<double>(3 * n);
+    c->v_alt = m.alloc<double>(3 * n); c->o_alt = m.alloc<double>(3 * n);
+    if (c->mixed()) {
+        c->f32 = m.alloc<float>(3 * n); c->t32 = m.alloc<float>(3 * n);
+        c->f32_alt = m.alloc<float>(3 * n); c->t32_alt = m.alloc<float>(3 * n);
+        c->fe32 = m.alloc<float>(3 * n); c->te32 = m.alloc<float>(3 * n);
+    } else {
+        c->f64 = m.alloc<double>(3 * n); c->t64 = m.alloc<double>(3 * n);
+        c->f64_alt = m.alloc<double>(3 * n); c->t64_alt = m.alloc<double>(3 * n);
+        c->fe = m.alloc<double>(3 * n); c->te = m.alloc<double>(3 * n);
+    }
+    c->e = m.alloc<double>(n); c->w = m.alloc<double>(n);
+    c->rb = m.alloc<float>(3 * n);
+    c->anc = m.alloc<uint32_t>(n);
+    c->type = m.alloc<uint8_t>(n); c->type_alt = m.alloc<uint8_t>(n);
+    c->id = m.alloc<uint32_t>(n); c->id_alt = m.alloc<uint32_t>(n);
+    c->slot = m.alloc<uint32_t>(n); c->slot_alt = m.alloc<uint32_t>(n);
+    c->inv = m.alloc<uint32_t>(n);
+    c->keys = m.alloc<uint32_t>(n); c->vals = m.alloc<uint32_t>(n);
+    c->keys_alt = m.alloc<uint32_t>(n); c->vals_alt = m.alloc<uint32_t>(n);
+    c->sort_temp = m.alloc<char>(radix_temp_bytes(n));
+    c->nnbr = m.alloc<uint32_t>(n);
+    c->nbnd = m.alloc<uint8_t>(n);
+    c->flags = m.alloc<unsigned>(8);
+    c->iflags = m.alloc<int>(8);
+    c->red = m.alloc<double>(reduce_blocks(n) + 8);
+    c->scal = m.alloc<double>(16);
+    c->bbox = m.alloc<double>(6 * 257);
+    c->stage = m.alloc<double>(3 * n);
+    c->stagef = m.alloc<float>(3 * n);
+    c->f_ext = m.alloc<double>(3 * n);
+    ORBC_HIP_CHECK(hipMemsetAsync(c->flags, 0, 8 * sizeof(unsigned), c->stream));
+    ORBC_HIP_CHECK(hipMemsetAsync(c->iflags, 0, 8 * sizeof(int), c->stream));
+    ORBC_HIP_CHECK(hipMemsetAsync(c->scal, 0, 16 * sizeof(double), c->stream));
+    ORBC_HIP_CHECK(hipMemsetAsync(c->e, 0, n * sizeof(double), c->stream));
+    ORBC_HIP_CHECK(hipMemsetAsync(c->w, 0, n * sizeof(double), c->stream));
+    ORBC_HIP_CHECK(hipMemsetAsync(c->nbnd, 0, n, c->stream));
+    ORBC_HIP_CHECK(hipMemsetAsync(c->f_ext, 0, 3 * n * sizeof(double), c->stream));
+    if (c->mixed()) {
+        ORBC_HIP_CHECK(hipMemsetAsync(c->f32, 0, 3 * n * sizeof(float), c->stream));
+        ORBC_HIP_CHECK(hipMemsetAsync(c->t32, 0, 3 * n * sizeof(float), c->stream));
+    } else {
+        ORBC_HIP_CHECK(hipMemsetAsync(c->f64, 0, 3 * n * sizeof(double), c->stream));
+        ORBC_HIP_CHECK(hipMemsetAsync(c->t64, 0, 3 * n * sizeof(double), c->stream));
+    }
+}
+
+void upload_bonds(orbc_ctx *c) {
+    // static per-particle bond lists by upload slot, both ends, ordered by
+    // (partner slot, type) so that the summation order is fixed
+    std::unordered_map<uint32_t, uint32_t> id2slot;
+    id2slot.reserve((size_t)c->np * 2 + 1);
+    for (long long u = 0; u < c->np; ++u) id2slot[c->sp[1].id[u]] = (uint32_t)(c->nl + u);
+    for (long long u = 0; u < c->nl; ++u) id2slot[c->sp[0].id[u]] = (uint32_t)u;
+    const long long nb = (long long)c->b_type.size();
+    std::vector<std::vector<std::pair<uint32_t, uint8_t>>> lists((size_t)c->n);
+    for (long long k = 0; k < nb; ++k) {
+        auto a = id2slot.find(c->b_i[k]), b = id2slot.find(c->b_j[k]);
+        if (a == id2slot.end() || b == id2slot.end()) throw std::runtime_error("bond refers to an unknown particle id");
+        if (c->b_type[k] < 0 || c->b_type[k] >= NBT) throw std::runtime_error("bond type out of range");
+        lists[a->second].push_back({b->second, (uint8_t)c->b_type[k]});
+        lists[b->second].push_back({a->second, (uint8_t)c->b_type[k]});
+    }
+    std::vector<uint32_t> start((size_t)c->n + 1, 0), partner;
+    std::vector<uint8_t> bt;
+    partner.reserve((size_t)(2 * nb));
+    bt.reserve((size_t)(2 * nb));
+    int kb = 0;
+    for (long long s = 0; s < c->n; ++s) {
+        auto &L = lists[s];
+        std::sort(L.begin(), L.end());
+        start[s] = (uint32_t)partner.size();
+        for (auto &p : L) { partner.push_back(p.first); bt.push_back(p.second); }
+        kb = std::max<int>(kb, (int)L.size());
+    }
+    start[c->n] = (uint32_t)partner.size();
+    if (kb > 255) throw std::runtime_error("more than 255 bonds on one particle");
+    c->mem.release(c->bstart); c->mem.release(c->bpartner); c->mem.release(c->btype_d); c->mem.release(c->bnd);
+    c->bstart = c->mem.alloc<uint32_t>(start.size());
+    c->bpartner = c->mem.alloc<uint32_t>(partner.size());
+    c->btype_d = c->mem.alloc<uint8_t>(bt.size());
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->bstart, start.data(), start.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!partner.empty()) {
+        ORBC_HIP_CHECK(hipMemcpyAsync(c->bpartner, partner.data(), partner.size() * 4, hipMemcpyHostToDevice, c->stream));
+        ORBC_HIP_CHECK(hipMemcpyAsync(c->btype_d, bt.data(), bt.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    c->kbmax = kb;
+    c->nbonds = nb;
+    c->bnd = kb > 0 ? c->mem.alloc<uint32_t>((size_t)kb * c->stride) : nullptr;
+    sync(c);
+}
+
+void commit(orbc_ctx *c) {
+    if (c->live) {
+        voronoi_free(c);
+        c->mem.free_all();
+        c->live = false;
+    }
+    c->nl = c->sp[0].n;
+    c->np = c->sp[1].n;
+    c->n = c->nl + c->np;
+    if (c->n <= 0) throw std::runtime_error("no particles uploaded");
+    if (c->n >= (1LL << 30)) throw std::runtime_error("more than 2^30 particles");
+    c->stride = (c->n + 63) / 64 * 64;
+    alloc_store(c);
+    const long long n = c->n;
+    std::vector<double> hx(3 * n), hv(3 * n), hn(3 * n), ho(3 * n);
+    std::vector<uint8_t> ht(n);
+    std::vector<uint32_t> hid(n), hslot(n);
+    long long k = 0;
+    for (int s = 0; s < 2; ++s) {
+        const Species &S = c->sp[s];
+        for (long long u = 0; u < S.n; ++u, ++k) {
+            for (int d = 0; d < 3; ++d) {
+                hx[3 * k + d] = S.x[3 * u + d];
+                hv[3 * k + d] = S.v.empty() ? 0.0 : S.v[3 * u + d];
+                hn[3 * k + d] = S.nrm[3 * u + d];
+                ho[3 * k + d] = S.o.empty() ? 0.0 : S.o[3 * u + d];
+            }
+            const int t = s == 0 ? 0 : S.type[u];
+            if (t < 0 || t >= NT || (s == 1 && t == 0)) throw std::runtime_error("protein type must be 1..5");
+            ht[k] = (uint8_t)t;
+            hid[k] = S.id[u];
+            hslot[k] = (uint32_t)k;
+        }
+    }
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->x[0], hx.data(), 24 * n, hipMemcpyHostToDevice, c->stream));
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->v, hv.data(), 24 * n, hipMemcpyHostToDevice, c->stream));
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->nrm[0], hn.data(), 24 * n, hipMemcpyHostToDevice, c->stream));
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->o, ho.data(), 24 * n, hipMemcpyHostToDevice, c->stream));
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->type, ht.data(), n, hipMemcpyHostToDevice, c->stream));
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->id, hid.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->slot, hslot.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+    const double z = c->cfg.zeta;
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->scal, &z, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->cur = 0;
+    c->live = true;
+    c->nh_started = false;
+    c->nbonds = 0; c->kbmax = 0;
+    if (!c->b_type.empty()) upload_bonds(c);
+    c->bonds_dirty = false;
+    c->need_rebuild = true;
+    c->kmax = 0;
+    sync(c);
+    rebuild(c);
+}
+
+}  // namespace
+
+namespace orbc {
+
+// list cutoffs and search rings for the current skin
+void set_cutoffs(orbc_ctx *c, double cs) {
+    float cut2[36];
+    int rings[NT];
+    const double skin = c->cfg.skin;
+    for (int a = 0; a < NT; ++a) {
+        double rmax = 0;
+        for (int b = 0; b < NT; ++b) {
+            const double rc = ff_max_cut(a, b);
+            cut2[a * NT + b] = rc > 0 ? (float)((rc + skin) * (rc + skin)) : -1.0f;
+            if (rc > 0) rmax = std::max(rmax, rc + skin);
+        }
+        rings[a] = std::max(1, (int)std::ceil(rmax / cs - 1e-12));
+    }
+    set_list_cutoffs(cut2, rings, c->stream);
+}
+
+void rebuild(orbc_ctx *c) {
+    const long long n = c->n;
+    hipStream_t s = c->stream;
+    // 1. bounding box
+    launch_bbox(c->x[c->cur], n, c->bbox, s);
+    double bb[6];
+    ORBC_HIP_CHECK(hipMemcpyAsync(bb, c->bbox, sizeof(bb), hipMemcpyDeviceToHost, s));
+    sync(c);
+    for (int d = 0; d < 6; ++d)
+        if (!std::isfinite(bb[d])) throw std::runtime_error("non-finite particle position at neighbour rebuild");
+    // 2. binning grid: cell edge = lipid cutoff + skin
+    const double cs = kCutLL + c->cfg.skin;
+    Grid g{};
+    g.cs = cs;
+    for (int d = 0; d < 3; ++d) {
+        g.lo[d] = bb[d] - 1e-6 * (1.0 + std::fabs(bb[d]));
+        g.dim[d] = std::max(1, (int)std::floor((bb[3 + d] - g.lo[d]) / cs) + 1);
+        g.bits[d] = ceil_log2(g.dim[d]);
+    }
+    if (g.bits[0] + g.bits[1] + g.bits[2] > 32) throw std::runtime_error("system extent exceeds the 32-bit Morton grid");
+    g.ncell = (long long)g.dim[0] * g.dim[1] * g.dim[2];
+    if (g.ncell > c->ncell_cap) {
+        c->mem.release(c->cstart); c->mem.release(c->cend);
+        c->ncell_cap = std::max(g.ncell, (long long)(1.25 * (double)g.ncell));
+        c->cstart = c->mem.alloc<uint32_t>(c->ncell_cap);
+        c->cend = c->mem.alloc<uint32_t>(c->ncell_cap);
+    }
+    c->grid = g;
+    set_cutoffs(c, cs);
+    // 3. anchor grid: smallest power-of-two spacing >= 1/4 whose anchors pack in 32 bits
+    Anchor ag{};
+    for (int k = -2; k < 40; ++k) {
+        const double h = std::ldexp(1.0, k);
+        int bits[3];
+        long long A0[3];
+        int tot = 0;
+        for (int d = 0; d < 3; ++d) {
+            A0[d] = (long long)std::floor(bb[d] / h) - 1;
+            const long long span = (long long)std::floor(bb[3 + d] / h) - A0[d] + 2;
+            bits[d] = std::max(1, ceil_log2(span));
+            tot += bits[d];
+        }
+        if (tot <= 32) {
+            ag.hd = h; ag.h = (float)h;
+            for (int d = 0; d < 3; ++d) ag.A0[d] = A0[d];
+            ag.shy = bits[0]; ag.shz = bits[0] + bits[1];
+            ag.mx = (1u << bits[0]) - 1u; ag.my = (1u << bits[1]) - 1u;
+            ag.mz = bits[2] >= 32 ? 0xffffffffu : ((1u << bits[2]) - 1u);
+            break;
+        }
+    }
+    if (ag.hd == 0) throw std::runtime_error("cannot pack anchors");
+    c->ag = ag;
+    // 4. Morton sort of the particles by grid cell
+    launch_keys(c->x[c->cur], n, g, c->keys, c->vals, s);
+    bool alt = false;
+    radix_sort_pairs(c->keys, c->vals, c->keys_alt, c->vals_alt, n, g.bits[0] + g.bits[1] + g.bits[2], c->sort_temp, s,
+                     &alt);
+    const uint32_t *perm = alt ? c->vals_alt : c->vals;
+    // 5. permute the store
+    PermuteArgs p{};
+    p.n = n; p.perm = perm;
+    const int nx = 1 - c->cur;
+    p.src3[0] = c->x[c->cur]; p.dst3[0] = c->x[nx];
+    p.src3[1] = c->nrm[c->cur]; p.dst3[1] = c->nrm[nx];
+    p.src3[2] = c->v; p.dst3[2] = c->v_alt;
+    p.src3[3] = c->o; p.dst3[3] = c->o_alt;
+    if (c->mixed()) {
+        p.n3 = 4;
+        p.srcf3[0] = c->f32; p.dstf3[0] = c->f32_alt;
+        p.srcf3[1] = c->t32; p.dstf3[1] = c->t32_alt;
+        p.nf3 = 2;
+    } else {
+        p.src3[4] = c->f64; p.dst3[4] = c->f64_alt;
+        p.src3[5] = c->t64; p.dst3[5] = c->t64_alt;
+        p.n3 = 6; p.nf3 = 0;
+    }
+    p.type_src = c->type; p.type_dst = c->type_alt;
+    p.id_src = c->id; p.id_dst = c->id_alt;
+    p.slot_src = c->slot; p.slot_dst = c->slot_alt;
+    p.aff_src = c->voro.aff; p.aff_dst = c->voro.aff_alt;
+    launch_permute(p, s);
+    c->cur = nx;
+    std::swap(c->v, c->v_alt); std::swap(c->o, c->o_alt);
+    std::swap(c->f32, c->f32_alt); std::swap(c->t32, c->t32_alt);
+    std::swap(c->f64, c->f64_alt); std::swap(c->t64, c->t64_alt);
+    std::swap(c->type, c->type_alt); std::swap(c->id, c->id_alt); std::swap(c->slot, c->slot_alt);
+    if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
+    launch_inverse(c->slot, c->inv, n, s);
+    // 6. mirrors, anchors, build-time offsets
+    launch_refresh(n, c->x[c->cur], c->nrm[c->cur], c->type, ag, c->xq[c->cur], c->nq[c->cur], c->rb, c->anc, s);
+    // 7. cell ranges and the Verlet list
+    launch_cell_ranges(c->x[c->cur], n, g, c->cstart, c->cend, s);
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        if (c->kmax == 0) {
+            c->kmax = 48;
+            c->nbr = c->mem.alloc<uint32_t>((size_t)c->kmax * c->stride);
+        }
+        NlistArgs a{};
+        a.n = n; a.stride = c->stride; a.x = c->x[c->cur]; a.type = c->type; a.g = g;
+        for (int d = 0; d < 3; ++d) { a.box.lo[d] = c->cfg.box_lo[d]; a.box.hi[d] = c->cfg.box_hi[d];
+                                      a.box.L[d] = c->cfg.box_hi[d] - c->cfg.box_lo[d]; a.box.per[d] = 0; }
+        a.cstart = c->cstart; a.cend = c->cend; a.nbr = c->nbr; a.nnbr = c->nnbr; a.kmax = c->kmax;
+        a.max_count = c->flags + 2; a.skin = c->cfg.skin;
+        if (c->cfg.neighbor_mode == ORBC_NEIGHBOR_VORONOI && c->voro.aff) {
+            a.aff = c->voro.aff; a.centroids = c->voro.c;
+        }
+        zero_flag(c, 2);
+        launch_nlist(a, s);
+        const unsigned mc = read_flag(c, 2);
+        if ((int)mc <= c->kmax) break;
+        c->mem.release(c->nbr);
+        c->kmax = (int)mc + 16;
+        c->nbr = c->mem.alloc<uint32_t>((size_t)c->kmax * c->stride);
+    }
+    // 8. bond lists
+    if (c->nbonds > 0)
+        launch_bondlist(n, c->stride, c->slot, c->inv, c->bstart, c->bpartner, c->btype_d, c->bnd, c->nbnd, s);
+    zero_flag(c, 0);
+    c->need_rebuild = false;
+    c->moved = false;
+    c->since_build = 0;
+    c->rebuilds++;
+}
+
+void ensure_live(orbc_ctx *c) {
+    if (!c->live) commit(c);
+    if (c->bonds_dirty) { upload_bonds(c); c->bonds_dirty = false; c->need_rebuild = true; }
+    if (c->need_rebuild) rebuild(c);
+}
+
+// make the list valid for the current positions (skin criterion)
+void ensure_list(orbc_ctx *c) {
+    ensure_live(c);
+    if (c->moved) {
+        zero_flag(c, 0);
+        launch_disp(c->n, c->x[c->cur], c->anc, c->rb, c->ag, c->flags, c->stream);
+        c->moved = false;
+    }
+    const unsigned bits = read_flag(c, 0);
+    float d2;
+    std::memcpy(&d2, &bits, 4);
+    const double half = 0.5 * c->cfg.skin;
+    if (!(d2 <= half * half)) rebuild(c);
+}
+
+}  // namespace orbc
+
+// =================================================================== C ABI
+extern "C" {
+
+void orbc_config_default(orbc_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->precision = ORBC_MIXED;
+    cfg->kBT = 0.22;            // src/runtime_parameter.h:49
+    cfg->eta = 0.01;            // :50
+    cfg->dt = 1e-2;             // :55
+    cfg->rho = 1.05;            // :48
+    cfg->voronoi_cell_size = 14;// :47
+    cfg->stray_tolerance = 2.5; // :51
+    cfg->freq_voronoi = 2;      // :59
+    cfg->freq_sort_ctrd = 24;   // :60
+    cfg->freq_sort_bond = 120;  // :61
+    cfg->freq_cleanup = 60;     // :62
+    cfg->seed = 0xBAD5EED;      // :64
+    cfg->Q = 0.0;
+    cfg->zeta = 0.0;
+    for (int d = 0; d < 3; ++d) { cfg->box_lo[d] = -1000; cfg->box_hi[d] = 1000; }   // :111-115
+    cfg->skin = 0.4;
+    cfg->rebuild_period = 0;
+    cfg->neighbor_mode = ORBC_NEIGHBOR_EXACT;
+    cfg->thermostat = ORBC_RUN_LANGEVIN;
+    cfg->volume_target = 0.0;
+    cfg->volume_strength = 0.05;   // src/openrbc.cpp:229
+    cfg->device = 0;
+    cfg->block_size = 0;
+}
+
+int orbc_create(const orbc_config *cfg, orbc_ctx **out) {
+    if (!out) return ORBC_ERR_INVALID;
+    *out = nullptr;
+    orbc_config c0;
+    orbc_config_default(&c0);
+    const orbc_config &cfg_ = cfg ? *cfg : c0;
+    if (cfg_.dt <= 0 || cfg_.skin <= 0 || cfg_.kBT < 0 || cfg_.eta < 0) return ORBC_ERR_INVALID;
+    if (cfg_.precision != ORBC_FP64 && cfg_.precision != ORBC_MIXED) return ORBC_ERR_INVALID;
+    // src/runtime_parameter.h:118-120 (reported, not exit(0))
+    if (cfg_.freq_voronoi <= 0 || cfg_.freq_sort_ctrd % cfg_.freq_voronoi || cfg_.freq_sort_bond % cfg_.freq_voronoi ||
+        cfg_.freq_cleanup % cfg_.freq_voronoi)
+        return ORBC_ERR_INVALID;
+    orbc_ctx *c = new orbc_ctx();
+    c->cfg = cfg_;
+    hipError_t e = hipSetDevice(cfg_.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->pinned, 64, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->pinned_d, 64 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        delete c;
+        return ORBC_ERR_HIP;
+    }
+    ff_fill(c->ffd, c->fff);
+    try {
+        upload_ff(c->ffd, c->fff, c->stream);
+    } catch (const HipError &) {
+        delete c;
+        return ORBC_ERR_HIP;
+    }
+    *out = c;
+    return ORBC_OK;
+}
+
+int orbc_destroy(orbc_ctx *c) {
+    if (!c) return ORBC_ERR_INVALID;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    voronoi_free(c);
+    c->mem.free_all();
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->pinned_d) (void)hipHostFree(c->pinned_d);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ORBC_OK;
+}
+
+const char *orbc_last_error(const orbc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int orbc_get_config(const orbc_ctx *c, orbc_config *cfg) {
+    if (!c || !cfg) return ORBC_ERR_INVALID;
+    *cfg = c->cfg;
+    return ORBC_OK;
+}
+
+int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const double *v, const double *nrm,
+                   const double *o, const int *type, const unsigned *id) {
+    return guarded(c, [&] {
+        if (species != ORBC_LIPID && species != ORBC_PROTEIN) return fail(c, ORBC_ERR_INVALID, "bad species");
+        if (n < 0 || (n > 0 && (!x || !nrm || !id))) return fail(c, ORBC_ERR_INVALID, "x, nrm and id are required");
+        if (species == ORBC_PROTEIN && n > 0 && !type) return fail(c, ORBC_ERR_INVALID, "protein type is required");
+        Species &S = c->sp[species];
+        const bool same = c->live && S.n == n;
+        S.n = n;
+        S.x.assign(x, x + 3 * n);
+        S.nrm.assign(nrm, nrm + 3 * n);
+        if (v) S.v.assign(v, v + 3 * n); else S.v.assign(3 * n, 0.0);
+        if (o) S.o.assign(o, o + 3 * n); else S.o.assign(3 * n, 0.0);
+        S.type.assign(n, 0);
+        if (species == ORBC_PROTEIN)
+            for (long long k = 0; k < n; ++k) {
+                if (type[k] < 1 || type[k] >= NT) return fail(c, ORBC_ERR_INVALID, "protein type must be 1..5");
+                S.type[k] = type[k];
+            }
+        S.id.assign(id, id + n);
+        if (same && n > 0) {
+            // update in place: upload in slot order, gather into device order
+            const long long off = species == ORBC_LIPID ? 0 : c->nl;
+            double *fields[4] = {c->x[c->cur], c->v, c->nrm[c->cur], c->o};
+            const std::vector<double> *src[4] = {&S.x, &S.v, &S.nrm, &S.o};
+            for (int f = 0; f < 4; ++f) {
+                // full upload-order image: fetch current, overwrite this species, gather back
+                launch_scatter_slots(c->n, c->slot, fields[f], c->stage, c->stream);
+                ORBC_HIP_CHECK(hipMemcpyAsync(c->stage + 3 * off, src[f]->data(), 24 * n, hipMemcpyHostToDevice, c->stream));
+                launch_gather_slots(c->n, c->slot, c->stage, fields[f], c->stream);
+            }
+            launch_refresh(c->n, c->x[c->cur], c->nrm[c->cur], c->type, c->ag, c->mixed() ? c->xq[c->cur] : nullptr,
+                           c->mixed() ? c->nq[c->cur] : nullptr, c->stagef, c->anc, c->stream);
+            c->need_rebuild = true;
+            sync(c);
+        } else if (c->live) {
+            c->live = false;   // counts changed: re-commit on next use
+            voronoi_free(c);
+            c->mem.free_all();
+        }
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_set_bonds(orbc_ctx *c, long long nb, const int *type, const unsigned *id_i, const unsigned *id_j) {
+    return guarded(c, [&] {
+        if (nb < 0 || (nb > 0 && (!type || !id_i || !id_j))) return fail(c, ORBC_ERR_INVALID, "bad bond arrays");
+        c->b_type.assign(type, type + nb);
+        c->b_i.assign(id_i, id_i + nb);
+        c->b_j.assign(id_j, id_j + nb);
+        c->bonds_dirty = c->live;
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_get_state(orbc_ctx *c, int species, double *x, double *v, double *nrm, double *o) {
+    return guarded(c, [&] {
+        if (species != ORBC_LIPID && species != ORBC_PROTEIN) return fail(c, ORBC_ERR_INVALID, "bad species");
+        const Species &S = c->sp[species];
+        if (!c->live) {
+            if (x) std::memcpy(x, S.x.data(), 24 * S.n);
+            if (v) std::memcpy(v, S.v.data(), 24 * S.n);
+            if (nrm) std::memcpy(nrm, S.nrm.data(), 24 * S.n);
+            if (o) std::memcpy(o, S.o.data(), 24 * S.n);
+            return (int)ORBC_OK;
+        }
+        const long long off = species == ORBC_LIPID ? 0 : c->nl;
+        const long long n = species == ORBC_LIPID ? c->nl : c->np;
+        const double *fields[4] = {c->x[c->cur], c->v, c->nrm[c->cur], c->o};
+        double *outs[4] = {x, v, nrm, o};
+        for (int f = 0; f < 4; ++f) {
+            if (!outs[f] || n == 0) continue;
+            launch_scatter_slots(c->n, c->slot, fields[f], c->stage, c->stream);
+            ORBC_HIP_CHECK(hipMemcpyAsync(outs[f], c->stage + 3 * off, 24 * n, hipMemcpyDeviceToHost, c->stream));
+            sync(c);
+        }
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_get_forces(orbc_ctx *c, int species, double *f, double *t, double *e, double *w) {
+    return guarded(c, [&] {
+        if (species != ORBC_LIPID && species != ORBC_PROTEIN) return fail(c, ORBC_ERR_INVALID, "bad species");
+        if (!c->live) return fail(c, ORBC_ERR_STATE, "no device state");
+        const long long off = species == ORBC_LIPID ? 0 : c->nl;
+        const long long n = species == ORBC_LIPID ? c->nl : c->np;
+        if (n == 0) return (int)ORBC_OK;
+        for (int k = 0; k < 2; ++k) {
+            double *out = k == 0 ? f : t;
+            if (!out) continue;
+            if (c->mixed()) launch_scatter_slots_f(c->n, c->slot, k == 0 ? c->f32 : c->t32, c->stage, c->stream);
+            else launch_scatter_slots(c->n, c->slot, k == 0 ? c->f64 : c->t64, c->stage, c->stream);
+            ORBC_HIP_CHECK(hipMemcpyAsync(out, c->stage + 3 * off, 24 * n, hipMemcpyDeviceToHost, c->stream));
+            sync(c);
+        }
+        for (int k = 0; k < 2; ++k) {
+            double *out = k == 0 ? e : w;
+            if (!out) continue;
+            launch_scatter_slots1(c->n, c->slot, k == 0 ? c->e : c->w, c->stage, c->stream);
+            ORBC_HIP_CHECK(hipMemcpyAsync(out, c->stage + off, 8 * n, hipMemcpyDeviceToHost, c->stream));
+            sync(c);
+        }
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_set_forces(orbc_ctx *c, int species, const double *f, const double *t) {
+    return guarded(c, [&] {
+        if (species != ORBC_LIPID && species != ORBC_PROTEIN) return fail(c, ORBC_ERR_INVALID, "bad species");
+        ensure_live(c);
+        const long long off = species == ORBC_LIPID ? 0 : c->nl;
+        const long long n = species == ORBC_LIPID ? c->nl : c->np;
+        if (n == 0) return (int)ORBC_OK;
+        for (int k = 0; k < 2; ++k) {
+            const double *src = k == 0 ? f : t;
+            if (!src) continue;
+            if (c->mixed()) {
+                launch_scatter_slots_f(c->n, c->slot, k == 0 ? c->f32 : c->t32, c->stage, c->stream);
+                ORBC_HIP_CHECK(hipMemcpyAsync(c->stage + 3 * off, src, 24 * n, hipMemcpyHostToDevice, c->stream));
+                launch_gather_slots_f(c->n, c->slot, c->stage, k == 0 ? c->f32 : c->t32, c->stream);
+            } else {
+                double *dst = k == 0 ? c->f64 : c->t64;
+                launch_scatter_slots(c->n, c->slot, dst, c->stage, c->stream);
+                ORBC_HIP_CHECK(hipMemcpyAsync(c->stage + 3 * off, src, 24 * n, hipMemcpyHostToDevice, c->stream));
+                launch_gather_slots(c->n, c->slot, c->stage, dst, c->stream);
+            }
+        }
+        sync(c);
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_build_neighbors(orbc_ctx *c) {
+    return guarded(c, [&] {
+        ensure_live(c);
+        rebuild(c);
+        sync(c);
+        return (int)ORBC_OK;
+    });
+}
+
+static void clear_ft(orbc_ctx *c, bool energy) {
+    const long long n = c->n;
+    if (c->mixed()) {
+        ORBC_HIP_CHECK(hipMemsetAsync(c->f32, 0, 12 * n, c->stream));
+        ORBC_HIP_CHECK(hipMemsetAsync(c->t32, 0, 12 * n, c->stream));
+    } else {
+        ORBC_HIP_CHECK(hipMemsetAsync(c->f64, 0, 24 * n, c->stream));
+        ORBC_HIP_CHECK(hipMemsetAsync(c->t64, 0, 24 * n, c->stream));
+    }
+    if (energy) {
+        ORBC_HIP_CHECK(hipMemsetAsync(c->e, 0, 8 * n, c->stream));
+        ORBC_HIP_CHECK(hipMemsetAsync(c->w, 0, 8 * n, c->stream));
+    }
+}
+
+int orbc_compute_pairwise(orbc_ctx *c, unsigned flags) {
+    return guarded(c, [&] {
+        ensure_list(c);
+        const bool energy = flags & ORBC_F_ENERGY;
+        if (flags & ORBC_F_ZERO_FIRST) clear_ft(c, energy);
+        launch_forces(force_args(c), c->mixed(), energy, true, false, c->stream);
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_compute_bonded(orbc_ctx *c, unsigned flags) {
+    return guarded(c, [&] {
+        ensure_live(c);
+        const bool energy = flags & ORBC_F_ENERGY;
+        if (flags & ORBC_F_ZERO_FIRST) clear_ft(c, energy);
+        if (c->nbonds > 0) launch_forces(force_args(c), c->mixed(), energy, false, true, c->stream);
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_integrate(orbc_ctx *c, int kernel) {
+    return guarded(c, [&] {
+        if (kernel < ORBC_CLEAR_FORCE || kernel > ORBC_BOUNCE_BACK) return fail(c, ORBC_ERR_INVALID, "bad kernel");
+        ensure_live(c);
+        IntegArgs a = integ_args(c);
+        launch_integrate(kernel, a, c->mixed(), c->stream);
+        if (kernel == ORBC_NH_INITIAL_BOUNCE_CLEAR || kernel == ORBC_NH_POST_TORQUE_FINAL)
+            launch_zeta_update(c->red, reduce_blocks(c->n), c->scal, c->cfg.dt, nh_Q(c), 1.5 * (double)c->n * c->cfg.kBT,
+                               c->stream);
+        if (kernel == ORBC_VERLET_LANGEVIN) c->step++;
+        if (kernel == ORBC_VERLET_LANGEVIN || kernel == ORBC_NH_INITIAL_BOUNCE_CLEAR || kernel == ORBC_BOUNCE_BACK)
+            c->moved = true;
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_compute_temperature(orbc_ctx *c, double *T) {
+    return guarded(c, [&] {
+        if (!T) return fail(c, ORBC_ERR_INVALID, "null T");
+        ensure_live(c);
+        launch_reduce_ke(c->v, c->type, c->n, c->red, c->stream);
+        launch_sum_partials(c->red, reduce_blocks(c->n), c->scal + 1, c->stream);
+        double ke;
+        ORBC_HIP_CHECK(hipMemcpyAsync(&ke, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        *T = 2.0 * ke / (3.0 * (double)c->n);
+        return (int)ORBC_OK;
+    });
+}
+
+static StepArgs step_args(orbc_ctx *c) {
+    StepArgs a{};
+    const int in = c->cur, out = 1 - c->cur;
+    a.n = c->n; a.stride = c->stride;
+    a.x_in = c->x[in]; a.n_in = c->nrm[in];
+    a.x_out = c->x[out]; a.n_out = c->nrm[out];
+    a.v = c->v; a.o = c->o;
+    a.xq_in = c->xq[in]; a.nq_in = c->nq[in];
+    a.xq_out = c->xq[out]; a.nq_out = c->nq[out];
+    a.type = c->type; a.id = c->id; a.anc = c->anc; a.rb = c->rb;
+    a.nbr = c->nbr; a.nnbr = c->nnbr;
+    a.bnd = c->nbonds ? c->bnd : nullptr;
+    a.nbnd = c->nbonds ? c->nbnd : nullptr;
+    a.has_bonds = c->nbonds > 0;
+    a.f_ext = nullptr;
+    a.anc_grid = c->ag;
+    a.dt = c->cfg.dt;
+    langevin_coeffs(c->cfg, a.gamma, a.sigma);
+    a.seed = c->cfg.seed;
+    a.step = (unsigned long long)c->step;
+    a.disp_max = c->flags;
+    a.bad = c->flags + 1;
+    a.skip = nullptr;
+    a.done = nullptr;
+    a.swizzle = 1;
+    return a;
+}
+
+int orbc_run(orbc_ctx *c, long long nsteps) {
+    return guarded(c, [&] {
+        if (nsteps < 0) return fail(c, ORBC_ERR_INVALID, "nsteps < 0");
+        ensure_list(c);
+        const double half2 = 0.25 * c->cfg.skin * c->cfg.skin;
+        if (c->cfg.thermostat == ORBC_RUN_LANGEVIN) {
+            for (long long s = 0; s < nsteps; ++s) {
+                StepArgs a = step_args(c);
+                launch_step_langevin(a, c->mixed(), c->stream);
+                c->cur = 1 - c->cur;
+                c->step++;
+                c->since_build++;
+                // skin criterion (or fixed period), checked after every step
+                bool rb = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
+                if (!rb) {
+                    const unsigned bits = read_flag(c, 0);
+                    float d2;
+                    std::memcpy(&d2, &bits, 4);
+                    rb = !(d2 <= half2);
+                    if (read_flag(c, 1)) return fail(c, ORBC_ERR_NAN, "non-finite state in orbc_run");
+                }
+                if (rb && s + 1 < nsteps) rebuild(c);
+                else if (rb) c->need_rebuild = true;
+            }
+        } else {
+            // Nose-Hoover, src/openrbc.cpp:176-243 with FUSED_INTEGRATOR
+            IntegArgs a;
+            const double Q = nh_Q(c), nkT = 1.5 * (double)c->n * c->cfg.kBT;
+            const long long nb = reduce_blocks(c->n);
+            auto forces = [&] {
+                clear_ft(c, false);
+                ForceArgs fa = force_args(c);
+                launch_forces(fa, c->mixed(), false, true, c->nbonds > 0, c->stream);
+            };
+            if (!c->nh_started) {
+                forces();
+                a = integ_args(c);
+                launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
+                launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
+                c->nh_started = true;
+            }
+            for (long long s = 0; s < nsteps; ++s) {
+                a = integ_args(c);
+                launch_integrate(ORBC_NH_INITIAL_BOUNCE_CLEAR, a, c->mixed(), c->stream);
+                launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
+                c->moved = true;
+                c->since_build++;
+                bool rb = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
+                if (rb) { rebuild(c); } else { ensure_list(c); }
+                forces();
+                a = integ_args(c);
+                launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
+                launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
+                c->step++;
+            }
+        }
+        sync(c);
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_minimize(orbc_ctx *c, long long nsteps, double dr_opt, double dn_opt) {
+    return guarded(c, [&] {
+        for (long long s = 0; s < nsteps; ++s) {
+            ensure_list(c);
+            clear_ft(c, false);
+            launch_forces(force_args(c), c->mixed(), false, true, c->nbonds > 0, c->stream);
+            IntegArgs a = integ_args(c);
+            a.dr_opt = dr_opt; a.dn_opt = dn_opt;
+            launch_integrate(ORBC_POST_TORQUE, a, c->mixed(), c->stream);
+            launch_minimize(a, c->mixed(), c->stream);
+            launch_integrate(ORBC_BOUNCE_BACK, a, c->mixed(), c->stream);
+            c->moved = true;
+        }
+        clear_ft(c, false);
+        sync(c);
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_get_observables(orbc_ctx *c, orbc_observables *obs) {
+    return guarded(c, [&] {
+        if (!obs) return fail(c, ORBC_ERR_INVALID, "null obs");
+        ensure_list(c);
+        std::memset(obs, 0, sizeof(*obs));
+        const long long nb = reduce_blocks(c->n);
+        launch_reduce_ke(c->v, c->type, c->n, c->red, c->stream);
+        launch_sum_partials(c->red, nb, c->scal + 1, c->stream);
+        // energy and virial into scratch force buffers (f,t untouched)
+        ForceArgs fa = force_args(c);
+        if (c->mixed()) {
+            ORBC_HIP_CHECK(hipMemsetAsync(c->fe32, 0, 12 * c->n, c->stream));
+            ORBC_HIP_CHECK(hipMemsetAsync(c->te32, 0, 12 * c->n, c->stream));
+            fa.f32 = c->fe32; fa.t32 = c->te32;
+        } else {
+            ORBC_HIP_CHECK(hipMemsetAsync(c->fe, 0, 24 * c->n, c->stream));
+            ORBC_HIP_CHECK(hipMemsetAsync(c->te, 0, 24 * c->n, c->stream));
+            fa.f64 = c->fe; fa.t64 = c->te;
+        }
+        ORBC_HIP_CHECK(hipMemsetAsync(c->e, 0, 8 * c->n, c->stream));
+        ORBC_HIP_CHECK(hipMemsetAsync(c->w, 0, 8 * c->n, c->stream));
+        launch_forces(fa, c->mixed(), true, true, c->nbonds > 0, c->stream);
+        launch_sum_array(c->e, c->n, c->red, c->stream);
+        launch_sum_partials(c->red, nb, c->scal + 2, c->stream);
+        launch_sum_array(c->w, c->n, c->red, c->stream);
+        launch_sum_partials(c->red, nb, c->scal + 3, c->stream);
+        double s[4];
+        ORBC_HIP_CHECK(hipMemcpyAsync(s, c->scal, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        obs->zeta = s[0];
+        obs->kinetic = s[1];
+        obs->temperature = 2.0 * s[1] / (3.0 * (double)c->n);
+        obs->potential = s[2];
+        obs->virial = s[3];
+        obs->n = c->n;
+        obs->step = c->step;
+        obs->rebuilds = c->rebuilds;
+        // directed list entries
+        std::vector<uint32_t> cnt(c->n);
+        ORBC_HIP_CHECK(hipMemcpyAsync(cnt.data(), c->nnbr, 4 * c->n, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        long long tot = 0;
+        for (auto v : cnt) tot += v;
+        obs->pairs = tot;
+        obs->anchor_h = c->ag.hd;
+        obs->skin = c->cfg.skin;
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_get_pairs(orbc_ctx *c, long long cap, unsigned *id_i, unsigned *id_j, long long *count) {
+    return guarded(c, [&] {
+        if (!count || cap < 0 || (cap > 0 && (!id_i || !id_j))) return fail(c, ORBC_ERR_INVALID, "bad arguments");
+        ensure_list(c);
+        unsigned long long *cnt_d = nullptr;
+        unsigned *di = nullptr, *dj = nullptr;
+        cnt_d = c->mem.alloc<unsigned long long>(1);
+        di = c->mem.alloc<unsigned>(cap > 0 ? cap : 1);
+        dj = c->mem.alloc<unsigned>(cap > 0 ? cap : 1);
+        ORBC_HIP_CHECK(hipMemsetAsync(cnt_d, 0, 8, c->stream));
+        launch_pairs_export(force_args(c), c->mixed(), di, dj, cnt_d, cap, c->stream);
+        unsigned long long tot = 0;
+        ORBC_HIP_CHECK(hipMemcpyAsync(&tot, cnt_d, 8, hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        const long long m = std::min<long long>((long long)tot, cap);
+        if (m > 0) {
+            ORBC_HIP_CHECK(hipMemcpyAsync(id_i, di, 4 * m, hipMemcpyDeviceToHost, c->stream));
+            ORBC_HIP_CHECK(hipMemcpyAsync(id_j, dj, 4 * m, hipMemcpyDeviceToHost, c->stream));
+        }
+        sync(c);
+        c->mem.release(cnt_d); c->mem.release(di); c->mem.release(dj);
+        *count = (long long)tot;
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_synchronize(orbc_ctx *c) {
+    return guarded(c, [&] {
+        sync(c);
+        return (int)ORBC_OK;
+    });
+}
+
+}  // extern "C"
+

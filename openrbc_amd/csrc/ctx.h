@@ -1,0 +1,154 @@
+// ctx.h -- the orbc_ctx structure and host helpers shared by api.cpp and
+// voronoi.cpp.  Internal; the public ABI is include/orbc.h.
+#pragma once
+
+#include <algorithm>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/orbc.h"
+#include "orbc_internal.h"
+
+using orbc::Anchor;
+using orbc::Box;
+using orbc::FFTab;
+using orbc::Grid;
+using orbc::HipError;
+
+struct DevArena {
+    std::vector<void *> ptrs;
+    template <class T>
+    T *alloc(size_t n) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        if (e != hipSuccess) throw HipError(e, "hipMalloc", __FILE__, __LINE__);
+        ptrs.push_back(p);
+        return static_cast<T *>(p);
+    }
+    void release(void *p) {
+        if (!p) return;
+        auto it = std::find(ptrs.begin(), ptrs.end(), p);
+        if (it != ptrs.end()) { (void)hipFree(p); ptrs.erase(it); }
+    }
+    void free_all() {
+        for (void *p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct Species {
+    long long n = 0;
+    std::vector<double> x, v, nrm, o;
+    std::vector<int> type;
+    std::vector<uint32_t> id;
+};
+
+struct orbc_ctx {
+    orbc_config cfg;
+    std::string err;
+    hipStream_t stream = nullptr;
+    DevArena mem;
+    FFTab<double> ffd;
+    FFTab<float> fff;
+
+    Species sp[2];
+    std::vector<int> b_type;
+    std::vector<uint32_t> b_i, b_j;
+    bool bonds_dirty = false;
+
+    // device store
+    bool live = false;
+    long long n = 0, nl = 0, np = 0, stride = 0;
+    double *x[2] = {nullptr, nullptr}, *nrm[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *v = nullptr, *o = nullptr, *v_alt = nullptr, *o_alt = nullptr;
+    double *f64 = nullptr, *t64 = nullptr, *f64_alt = nullptr, *t64_alt = nullptr;
+    float *f32 = nullptr, *t32 = nullptr, *f32_alt = nullptr, *t32_alt = nullptr;
+    double *e = nullptr, *w = nullptr;
+    double *fe = nullptr, *te = nullptr;     // scratch force buffers for energy evaluation
+    float *fe32 = nullptr, *te32 = nullptr;
+    double *f_ext = nullptr;                 // volume-constraint force for orbc_run
+    float4 *xq[2] = {nullptr, nullptr}, *nq[2] = {nullptr, nullptr};
+    float *rb = nullptr;
+    uint32_t *anc = nullptr;
+    uint8_t *type = nullptr, *type_alt = nullptr;
+    uint32_t *id = nullptr, *id_alt = nullptr, *slot = nullptr, *slot_alt = nullptr, *inv = nullptr;
+    uint32_t *keys = nullptr, *vals = nullptr, *keys_alt = nullptr, *vals_alt = nullptr;
+    void *sort_temp = nullptr;
+    uint32_t *cstart = nullptr, *cend = nullptr;
+    long long ncell_cap = 0;
+    uint32_t *nbr = nullptr, *nnbr = nullptr;
+    int kmax = 0;
+    uint32_t *bnd = nullptr;
+    uint8_t *nbnd = nullptr;
+    int kbmax = 0;
+    uint32_t *bstart = nullptr, *bpartner = nullptr;
+    uint8_t *btype_d = nullptr;
+    long long nbonds = 0;
+    unsigned *flags = nullptr;   // [0] disp_max bits [1] bad [2] max_count
+    int *iflags = nullptr;       // [0] skip [1] done
+    double *red = nullptr;       // reduction partials
+    double *scal = nullptr;      // [0] zeta [1..7] sums
+    double *bbox = nullptr;      // 6 + 6*256
+    double *stage = nullptr;     // 3n doubles, upload-order staging
+    float *stagef = nullptr;
+    unsigned *pinned = nullptr;  // host-pinned flags
+    double *pinned_d = nullptr;
+
+    Grid grid;
+    Anchor ag;
+    Box box;
+    bool need_rebuild = true;
+    bool moved = false;          // positions changed by a standalone kernel since the last check
+    long long step = 0, rebuilds = 0, last_pairs = 0, since_build = 0;
+    bool nh_started = false;
+
+    // Voronoi (k_voronoi / voronoi.cpp)
+    struct Voro {
+        long long ncell = 0;
+        int32_t *aff = nullptr, *aff_alt = nullptr;
+        float *c = nullptr;           // centroids
+        unsigned long long *sums = nullptr;   // 4 * ncell fixed-point sums
+        double *partial = nullptr;            // reduction partials over cells
+        double *normals = nullptr;    // 3 * ncell
+        uint32_t *ck = nullptr, *cv = nullptr, *ck2 = nullptr, *cv2 = nullptr;
+        uint32_t *cg_start = nullptr, *cg_end = nullptr, *cg_idx = nullptr;
+        long long cg_cap = 0;
+        void *temp = nullptr;
+        Grid cg;
+        long long updates = 0;
+    } voro;
+
+    bool mixed() const { return cfg.precision == ORBC_MIXED; }
+};
+
+
+namespace orbc {
+int fail(orbc_ctx *c, int code, const std::string &msg);
+void sync(orbc_ctx *c);
+unsigned read_flag(orbc_ctx *c, int k);
+void zero_flag(orbc_ctx *c, int k);
+void ensure_live(orbc_ctx *c);
+void ensure_list(orbc_ctx *c);
+void rebuild(orbc_ctx *c);
+void voronoi_free(orbc_ctx *c);
+void voronoi_rebuild_index(orbc_ctx *c);
+}  // namespace orbc
+
+template <class F>
+int orbc_guarded(orbc_ctx *c, F &&f) {
+    if (!c) return ORBC_ERR_INVALID;
+    try {
+        return f();
+    } catch (const orbc::HipError &h) {
+        char buf[512];
+        std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)h.err, hipGetErrorString(h.err),
+                      h.expr, h.file, h.line);
+        return orbc::fail(c, h.err == hipErrorOutOfMemory ? ORBC_ERR_OOM : ORBC_ERR_HIP, buf);
+    } catch (const std::exception &ex) {
+        return orbc::fail(c, ORBC_ERR_INVALID, ex.what());
+    }
+}

@@ -1,0 +1,251 @@
+// orbc_internal.h -- shared definitions of the gfx950 kernels and the host
+// side of liborbc_hip.so.  Not part of the public ABI (include/orbc.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace orbc {
+
+constexpr int NT = 6;     // particle types, src/forcefield_canonical.h:31-37
+constexpr int NBT = 4;    // bond types, src/forcefield_canonical.h:101-109
+constexpr int BLOCK = 256;
+constexpr uint32_t BOND_INDEX_MASK = 0x3fffffffu;
+
+// ---------------------------------------------------------------------------
+// Force-field constants, src/forcefield_canonical.h:24-28,40-109.
+// The lipid-lipid values are compile-time constants so the dominant pair type
+// needs no table lookups; the tables are filled from the same expressions.
+constexpr double kCutLL = 2.6, kReqLL = 1.1225, kEpsLL = 1.2, kAlphaLL = 1.55;
+constexpr double ipow_c(double b, int e) { return e == 0 ? 1.0 : b * ipow_c(b, e - 1); }
+constexpr double kRepLL = kEpsLL / ipow_c(kCutLL - kReqLL, 8);
+constexpr double kAttLL = -2.0 * kEpsLL / ipow_c(kCutLL - kReqLL, 4);
+constexpr double kCut2LL = kCutLL * kCutLL;
+
+template <typename R>
+struct FFTab {
+    R cut2_poly[36], cut_poly[36], rep[36], att[36], alpha[36];
+    R cut2_lj[36], lj1[36], lj2[36];
+    int aniso[36];
+    R mass[NT], inv_mass[NT];
+    R bond_r0[NBT], bond_K[NBT];
+};
+
+void ff_fill(FFTab<double> &d, FFTab<float> &f);          // host, ff.cpp
+double ff_max_cut(int a, int b);                            // rc(a,b), 0 if none
+
+// ---------------------------------------------------------------------------
+// Geometry of the FP32 mirrors: position = h * A + r, A an integer anchor per
+// axis fixed at the last neighbour build, r the FP32 offset.  A - A0 is packed
+// into 32 bits (bits[0] | bits[1] << shy | bits[2] << shz).
+struct Anchor {
+    double hd;       // anchor spacing (power of two)
+    float h;
+    long long A0[3]; // anchor origin
+    unsigned shy, shz, mx, my, mz;
+};
+
+// Uniform binning grid of the neighbour build (Morton-ordered cells).
+struct Grid {
+    double lo[3];
+    double cs;       // cell edge
+    int dim[3];
+    int bits[3];     // Morton bits per axis
+    long long ncell;
+};
+
+// Periodic box (PBC on an axis when per[d] != 0), used by the kernels.
+struct Box {
+    double lo[3], hi[3], L[3];
+    int per[3];
+};
+
+// Everything a step kernel reads or writes, by value in kernel arguments.
+struct StepArgs {
+    long long n;
+    long long stride;          // ELL column stride (>= n)
+    // FP64 state (x, n read from *_in and written to *_out; v, o in place)
+    const double *x_in, *n_in;
+    double *x_out, *n_out, *v, *o;
+    // FP32 mirrors (mixed precision)
+    const float4 *xq_in, *nq_in;
+    float4 *xq_out, *nq_out;
+    const uint8_t *type;
+    const uint32_t *id;
+    const uint32_t *anc;       // packed anchors (FP64 path)
+    const float *rb;           // offsets at the last build (skin check)
+    const uint32_t *nbr, *nnbr;
+    const uint32_t *bnd;
+    const uint8_t *nbnd;
+    int has_bonds;
+    const double *f_ext;       // extra per-particle force (volume constraint), or null
+    Anchor anc_grid;
+    double dt;
+    double gamma[NT], sigma[NT];
+    unsigned long long seed, step;
+    unsigned *disp_max;        // float bits of max squared displacement since build
+    unsigned *bad;             // nonzero if a non-finite value appeared
+    const int *skip;           // if nonnull and *skip: do nothing (graph replay guard)
+    int *done;                 // incremented once per executed step (block 0)
+    int swizzle;               // XCD-aware block remap
+};
+
+// Per-particle accumulators for the standalone force / bond / integrate API.
+struct ForceArgs {
+    long long n, stride;
+    const double *x, *nrm;
+    const float4 *xq, *nq;
+    const uint8_t *type;
+    const uint32_t *id;
+    const uint32_t *nbr, *nnbr;
+    const uint32_t *bnd;
+    const uint8_t *nbnd;
+    Anchor anc_grid;
+    float *f32, *t32;
+    double *f64, *t64;
+    double *e, *w;
+    int swizzle;
+};
+
+// ---------------------------------------------------------------------------
+// launchers (each .hip file owns its kernels)
+void upload_ff(const FFTab<double> &d, const FFTab<float> &f, hipStream_t s);
+
+// k_sort.hip
+size_t scan_temp_bytes(long long n);
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, long long n, void *temp, hipStream_t s);
+size_t radix_temp_bytes(long long n);
+void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, long long n,
+                      int bits, void *temp, hipStream_t s, bool *result_in_alt);
+
+// k_build.hip
+void launch_bbox(const double *x, long long n, double *out6, hipStream_t s);
+void launch_keys(const double *x, long long n, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s);
+struct PermuteArgs {
+    long long n;
+    const uint32_t *perm;
+    const double *src3[6]; double *dst3[6]; int n3;     // AoS double3 arrays
+    const float *srcf3[2]; float *dstf3[2]; int nf3;    // AoS float3 arrays
+    const uint8_t *type_src; uint8_t *type_dst;
+    const uint32_t *id_src; uint32_t *id_dst;
+    const uint32_t *slot_src; uint32_t *slot_dst;
+    const int32_t *aff_src; int32_t *aff_dst;
+};
+void launch_permute(const PermuteArgs &a, hipStream_t s);
+void launch_inverse(const uint32_t *slot, uint32_t *inv, long long n, hipStream_t s);
+void launch_refresh(long long n, const double *x, const double *nrm, const uint8_t *type, Anchor a,
+                    float4 *xq, float4 *nq, float *rb, uint32_t *anc, hipStream_t s);
+void launch_cell_ranges(const double *x, long long n, Grid g, uint32_t *cstart, uint32_t *cend, hipStream_t s);
+struct NlistArgs {
+    long long n, stride;
+    const double *x;
+    const uint8_t *type;
+    Grid g;
+    Box box;
+    const uint32_t *cstart, *cend;
+    uint32_t *nbr, *nnbr;
+    int kmax;
+    unsigned *max_count;
+    double skin;
+    const int32_t *aff;      // Voronoi filter (null in exact mode)
+    const float *centroids;
+};
+void launch_nlist(const NlistArgs &a, hipStream_t s);
+void set_list_cutoffs(const float *cut2_list, const int *rings, hipStream_t s);
+void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g,
+                 unsigned *disp_max, hipStream_t s);
+void launch_bondlist(long long n, long long stride, const uint32_t *slot, const uint32_t *inv,
+                     const uint32_t *bstart, const uint32_t *bpartner, const uint8_t *btype,
+                     uint32_t *bnd, uint8_t *nbnd, hipStream_t s);
+
+// k_force.hip
+void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool bond, hipStream_t s);
+void launch_step_langevin(const StepArgs &a, bool mixed, hipStream_t s);
+void launch_step_nh_second(const StepArgs &a, bool mixed, double *f64, double *t64, double *ke_partial,
+                           const double *zeta, hipStream_t s);
+long long step_blocks(long long n);
+void launch_pairs_export(const ForceArgs &a, bool mixed, unsigned *ids_i, unsigned *ids_j,
+                         unsigned long long *count, long long cap, hipStream_t s);
+
+// k_integrate.hip
+struct IntegArgs {
+    long long n;
+    double *x, *v, *nrm, *o;
+    float *f32, *t32;
+    double *f64, *t64;
+    const uint8_t *type;
+    const uint32_t *id;
+    const uint32_t *anc;
+    float4 *xq, *nq;   // mirrors refreshed in place when nonnull
+    Anchor anc_grid;
+    double dt;
+    double gamma[NT], sigma[NT];
+    unsigned long long seed, step;
+    double box_lo[3], box_hi[3];
+    const double *zeta;       // device scalar
+    double *ke_partial;       // per block
+    double dr_opt, dn_opt;
+};
+void launch_integrate(int kernel, const IntegArgs &a, bool mixed, hipStream_t s);
+void launch_minimize(const IntegArgs &a, bool mixed, hipStream_t s);
+void launch_reduce_ke(const double *v, const uint8_t *type, long long n, double *partial, hipStream_t s);
+void launch_sum_partials(const double *partial, long long nblocks, double *out, hipStream_t s);
+void launch_zeta_update(const double *partial, long long nblocks, double *zeta, double dt, double Q,
+                        double nkT15, hipStream_t s);
+void launch_sum_array(const double *a, long long n, double *partial, hipStream_t s);
+long long reduce_blocks(long long n);
+void launch_check_finite(const double *x, long long n3, unsigned *bad, hipStream_t s);
+void launch_scatter_slots(long long n, const uint32_t *slot, const double *src3, double *dst3, hipStream_t s);
+void launch_scatter_slots_f(long long n, const uint32_t *slot, const float *src3, double *dst3, hipStream_t s);
+void launch_scatter_slots1(long long n, const uint32_t *slot, const double *src, double *dst, hipStream_t s);
+void launch_gather_slots(long long n, const uint32_t *slot, const double *src3, double *dst3, hipStream_t s);
+void launch_gather_slots_f(long long n, const uint32_t *slot, const double *src3, float *dst3, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s);
+
+// k_voronoi.hip
+struct VoroArgs {
+    long long n;            // particles
+    long long ncell;
+    const double *x;
+    const double *nrm;
+    const uint8_t *type;
+    int32_t *aff;
+    float *centroids;
+    double mass[NT];
+    // centroid grid (for nearest / within queries)
+    Grid cg;
+    const uint32_t *cg_start, *cg_end, *cg_idx;
+};
+void launch_morton_ref(const float *pts, long long n, double bsize, uint32_t *keys, hipStream_t s);
+void launch_centroid_keys(const float *c, long long ncell, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s);
+void launch_cgrid_ranges(const uint32_t *sorted_keys, long long ncell, uint32_t *start, uint32_t *end,
+                         hipStream_t s);
+void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out,
+                    hipStream_t s);
+void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, int cap, int *counts,
+                   int *out, hipStream_t s);
+void launch_centroid_sums(const VoroArgs &a, unsigned long long *sums, hipStream_t s);
+void launch_centroid_finish(const unsigned long long *sums, long long ncell, float *centroids, hipStream_t s);
+void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals, double *partial, double *scal,
+                   double target, double strength, hipStream_t s);
+void launch_volume_push(const VoroArgs &a, const double *normals, const double *fpush, double *f_ext,
+                        double *f64, float *f32, hipStream_t s);
+
+}  // namespace orbc
+
+#define ORBC_HIP_CHECK(expr)                                                     \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess) throw orbc::HipError(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+namespace orbc {
+struct HipError {
+    hipError_t err;
+    const char *expr, *file;
+    int line;
+    HipError(hipError_t e, const char *x, const char *f, int l) : err(e), expr(x), file(f), line(l) {}
+};
+}  // namespace orbc

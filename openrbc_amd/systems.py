@@ -1,0 +1,170 @@
+"""Synthetic inputs: meshes and particle systems.
+
+* ``init_rbc``: mesh -> glycophorin, actin, band-3, spectrin and lipids with
+  the rules of src/init_rbc.h:30-337 (C++ in liborbc_hip, ``orbc_init_rbc``).
+* ``icosphere``: subdivided icosahedron (level 3: 642 V / 1920 E / 1280 F),
+  written like example-large's vert/bond/face arrays (0-based).
+* ``load_example_large``: the full-size Evans-Fung RBC mesh that OpenRBC ships
+  (example-large/rbc.{vert,bond,face}.txt, 9128 V), stored as
+  openrbc_amd/data/rbc_example_large.npz.
+* ``random_sphere``: init_random_sphere (src/init_random.h:27-66), lipids only.
+* ``planar_patch``: hexagonal lipid lattice (periodic patch workloads).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def load_example_large():
+    z = np.load(os.path.join(_DATA, "rbc_example_large.npz"))
+    return z["vert"].astype(np.float64), z["edge"].astype(np.int32), z["face"].astype(np.int32)
+
+
+def icosphere(level: int = 3, radius: float = 1.0):
+    """Icosphere mesh (vert (nv,3), edge (ne,2), face (nf,3)), faces CCW seen from outside."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = list(f)
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius
+    F = np.array(faces, dtype=np.int32)
+    E = set()
+    for a, b, c in F:
+        for p, q in ((a, b), (b, c), (c, a)):
+            E.add((min(p, q), max(p, q)))
+    return V, np.array(sorted(E), dtype=np.int32), F
+
+
+def init_rbc(vert, edge, face, rho: float = 1.05, seed: int = 1):
+    """Particle system on a triangle mesh (src/init_rbc.h rules).
+
+    Returns a dict with lipid/protein containers (FP64 x, n; ids) and bonds
+    (type, id_i, id_j), ids following the reference tags."""
+    from . import load_library
+    from . import _System  # noqa: F401
+    L = load_library()
+    v = np.ascontiguousarray(vert, dtype=np.float64)
+    e = np.ascontiguousarray(edge, dtype=np.int32)
+    f = np.ascontiguousarray(face, dtype=np.int32)
+    s = _system_struct()
+    rc = L.orbc_init_rbc(v.ctypes.data_as(C.POINTER(C.c_double)), len(v), e.ctypes.data_as(C.POINTER(C.c_int)),
+                         len(e), f.ctypes.data_as(C.POINTER(C.c_int)), len(f), float(rho), int(seed), C.byref(s))
+    if rc != 0:
+        raise RuntimeError(f"orbc_init_rbc failed ({rc})")
+    try:
+        nl, np_, nb = s.n_lipid, s.n_protein, s.n_bond
+
+        def arr(p, n, dt):
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True) if n > 0 else np.zeros(0, dt)
+
+        out = dict(
+            lipid_x=arr(s.lipid_x, 3 * nl, np.float64).reshape(-1, 3),
+            lipid_n=arr(s.lipid_n, 3 * nl, np.float64).reshape(-1, 3),
+            lipid_id=arr(s.lipid_id, nl, np.uint32),
+            protein_x=arr(s.protein_x, 3 * np_, np.float64).reshape(-1, 3),
+            protein_n=arr(s.protein_n, 3 * np_, np.float64).reshape(-1, 3),
+            protein_type=arr(s.protein_type, np_, np.int32),
+            protein_id=arr(s.protein_id, np_, np.uint32),
+            bond_type=arr(s.bond_type, nb, np.int32),
+            bond_i=arr(s.bond_i, nb, np.uint32),
+            bond_j=arr(s.bond_j, nb, np.uint32),
+            area=s.area,
+            clutter_capped=s.clutter_capped,
+        )
+    finally:
+        L.orbc_system_free(C.byref(s))
+    return out
+
+
+def _system_struct():
+    from . import _System
+    return _System()
+
+
+def random_sphere(r: float = 100.0, rho: float = 1.05, seed: int = 0):
+    """init_random_sphere: int(rho 4 pi r^2) lipids uniform on a sphere, n = x/|x|."""
+    rng = np.random.default_rng(seed)
+    n = int(rho * 4 * 3.14159265358 * r * r)
+    theta = 2 * np.pi * rng.random(n)
+    u = 2 * rng.random(n) - 1
+    s = np.sqrt(1 - u * u)
+    x = np.stack([r * s * np.cos(theta), r * s * np.sin(theta), r * u], 1)
+    return dict(lipid_x=x, lipid_n=x / np.linalg.norm(x, axis=1, keepdims=True),
+                lipid_id=np.arange(1, n + 1, dtype=np.uint32))
+
+
+def planar_patch(nx: int, ny: int, a: float = 1.0487, tilt: float = 0.02, jitter: float = 0.02, seed: int = 0):
+    """Hexagonal lipid lattice nx * ny in the z=0 plane, directors +z with
+    Gaussian tilt (radians) and positions jittered by jitter * a."""
+    rng = np.random.default_rng(seed)
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    x = a * (i + 0.5 * (j % 2))
+    y = a * (np.sqrt(3) / 2) * j
+    p = np.stack([x.ravel(), y.ravel(), np.zeros(x.size)], 1)
+    p[:, :2] += rng.normal(0, jitter * a, (len(p), 2))
+    n = np.zeros_like(p)
+    n[:, 2] = 1
+    n[:, :2] = rng.normal(0, tilt, (len(p), 2))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return dict(lipid_x=p, lipid_n=n, lipid_id=np.arange(1, len(p) + 1, dtype=np.uint32))
+
+
+def tilt_directors(n, sigma, rng):
+    """Rotate unit vectors by Gaussian tilt of width sigma (radians)."""
+    g = rng.normal(0, sigma, n.shape)
+    m = n + g - (g * n).sum(1, keepdims=True) * n
+    return m / np.linalg.norm(m, axis=1, keepdims=True)
+
+
+def maxwell(types, kT, rng):
+    """Maxwell-Boltzmann velocities at kT (masses of src/forcefield_canonical.h:40)."""
+    mass = np.array([1, 4, 4, 1, 10, 10], dtype=np.float64)[np.asarray(types)]
+    return rng.normal(0, 1, (len(mass), 3)) * np.sqrt(kT / mass)[:, None]
+
+
+def system_to_simulation(sys: dict, sim, kT: float | None = None, tilt: float = 0.0, seed: int = 0):
+    """Upload a system dict into a Simulation (optionally tilt directors and
+    draw Maxwell velocities at kT)."""
+    from . import LIPID, PROTEIN
+    rng = np.random.default_rng(seed)
+    ln = sys["lipid_n"]
+    if tilt > 0:
+        ln = tilt_directors(ln, tilt, rng)
+    nl = len(sys["lipid_x"])
+    lv = maxwell(np.zeros(nl, np.int32), kT, rng) if kT else None
+    sim.set_state(LIPID, sys["lipid_x"], ln, sys["lipid_id"], v=lv)
+    if "protein_x" in sys and len(sys["protein_x"]):
+        pv = maxwell(sys["protein_type"], kT, rng) if kT else None
+        sim.set_state(PROTEIN, sys["protein_x"], sys["protein_n"], sys["protein_id"], v=pv,
+                      type=sys["protein_type"])
+        if len(sys["bond_type"]):
+            sim.set_bonds(sys["bond_type"], sys["bond_i"], sys["bond_j"])
+    else:
+        sim.set_state(PROTEIN, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.uint32),
+                      type=np.zeros(0, np.int32))
+    return sim
