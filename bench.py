@@ -1,0 +1,294 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-timesteps/s of the per-timestep MD loop on one MI355X.
+
+Workload: the full-size RBC of OpenRBC's example-large mesh (Evans-Fung
+biconcave, 9,128 vertices) populated by the init_rbc rules: 2,639,690 lipids +
+565,816 proteins = 3,205,506 particles, 584,066 bonds; Langevin at kT=0.22,
+dt=0.01, eta=0.01, seed 7, Verlet list rebuilt on the skin criterion.  The
+mixed-precision step (FP32 pair/bond forces, FP64 state and reductions) is the
+headline ``value``; the plain FP64 drop-in is reported beside it with the
+force error of mixed vs FP64 on the same configuration.
+
+A "step" is one fused launch of forces + bonds + Langevin update over all
+particles (rebuilds, when the skin criterion fires, are inside the timed
+region).  Inputs are resident in HBM before timing starts.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+METRIC = "particle-timesteps/s on 1 MI355X, FP32-mixed vs FP64, at stated force error"
+
+
+def algorithmic_bytes(precision: str, n: int, e_pairs: int, n_bonds: int) -> float:
+    """Bytes per step the loop must move at minimum (state in, state out,
+    one read of each pair index and bond): per particle 288 B (mixed) / 336 B
+    (FP64) + 8 B per unordered pair + 12 B per bond."""
+    base = 288.0 if precision == "mixed" else 336.0
+    return base * n + 8.0 * e_pairs + 12.0 * n_bonds
+
+
+def build_system(seed: int = 7):
+    import openrbc_amd as orbc
+    v, e, f = orbc.load_example_large()
+    return orbc.init_rbc(v, e, f, rho=1.05, seed=seed)
+
+
+def upload(sim, sysd, state=None, seed=7, kT=0.22):
+    import openrbc_amd as orbc
+    from openrbc_amd.systems import maxwell
+    rng = np.random.default_rng(seed)
+    if state is None:
+        lv = maxwell(np.zeros(len(sysd["lipid_x"]), np.int32), kT, rng)
+        pv = maxwell(sysd["protein_type"], kT, rng)
+        state = dict(lx=sysd["lipid_x"], ln=sysd["lipid_n"], lv=lv, lo=None,
+                     px=sysd["protein_x"], pn=sysd["protein_n"], pv=pv, po=None)
+    sim.set_state(orbc.LIPID, state["lx"], state["ln"], sysd["lipid_id"], v=state["lv"], o=state["lo"])
+    sim.set_state(orbc.PROTEIN, state["px"], state["pn"], sysd["protein_id"], v=state["pv"], o=state["po"],
+                  type=sysd["protein_type"])
+    sim.set_bonds(sysd["bond_type"], sysd["bond_i"], sysd["bond_j"])
+
+
+def download(sim):
+    import openrbc_amd as orbc
+    L = sim.get_state(orbc.LIPID)
+    P = sim.get_state(orbc.PROTEIN)
+    return dict(lx=L["x"], ln=L["n"], lv=L["v"], lo=L["o"], px=P["x"], pn=P["n"], pv=P["v"], po=P["o"])
+
+
+def forces(sim):
+    import openrbc_amd as orbc
+    sim.compute_pairwise(zero_first=True)
+    sim.compute_bonded()
+    lf, lt = sim.get_forces(orbc.LIPID)
+    pf, pt = sim.get_forces(orbc.PROTEIN)
+    return np.vstack([lf, pf]), np.vstack([lt, pt])
+
+
+def force_error(sim32, sim64, ids):
+    """eps_F = max_i |F32_i - F64_i| / rms_j |F64_j| (and the same for the
+    director force t), plus the 99.9th percentile of the per-particle
+    relative error |dF_i| / max(|F64_i|, median |F64|).  Pairs whose cutoff
+    test comes out differently in FP32 and FP64 (r within rounding of a cutoff;
+    the WCA force jumps by 2.2e-3 at the reference's cut 1.1225 > 2^(1/6)) are
+    counted, and eps_F is also given without the particles of such pairs."""
+    f64, t64 = forces(sim64)
+    f32, t32 = forces(sim32)
+    rms_f = np.sqrt(np.mean(np.sum(f64 * f64, 1)))
+    rms_t = np.sqrt(np.mean(np.sum(t64 * t64, 1)))
+    dF = np.linalg.norm(f32 - f64, axis=1)
+    dT = np.linalg.norm(t32 - t64, axis=1)
+    fn = np.linalg.norm(f64, axis=1)
+    p32 = sim32.pairs(); p64 = sim64.pairs()
+    k32 = p32[:, 0] * (1 << 32) + p32[:, 1]; k64 = p64[:, 0] * (1 << 32) + p64[:, 1]
+    flip = np.setxor1d(k32, k64)
+    flip_ids = np.unique(np.concatenate([flip >> 32, flip & 0xffffffff]))
+    keep = ~np.isin(ids, flip_ids)
+    return dict(eps_F=float(dF.max() / rms_f), eps_T=float(dT.max() / rms_t),
+                p99_9_rel=float(np.percentile(dF / np.maximum(fn, np.median(fn)), 99.9)), rms_F=float(rms_f),
+                cutoff_flips=int(len(flip)), eps_F_no_flip=float(dF[keep].max() / rms_f),
+                eps_T_no_flip=float(dT[keep].max() / rms_t))
+
+
+def timed_run(sim, steps: int, warmup: int):
+    sim.run(warmup)
+    sim.synchronize()
+    sim.set_timing(True)
+    r0 = sim.observables()["rebuilds"]
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    tm = sim.timing()
+    obs = sim.observables()
+    sim.set_timing(False)
+    return t1 - t0, tm, obs, obs["rebuilds"] - r0
+
+
+def cpu_baseline(sysd, steps: int, n_lloyd: int, threads: int, timeout: int = 240):
+    """The OpenRBC reference itself (oracle/_ref, compiled in place from its
+    sources) on the same system, in a subprocess."""
+    from oracle import REF_SO
+    if not os.path.exists(REF_SO):
+        return None
+    import tempfile
+    path = os.path.join(tempfile.gettempdir(), f"orbc_cpu_sample_{os.getpid()}.npz")
+    np.savez(path, lx=sysd["lipid_x"].astype(np.float32), ln=sysd["lipid_n"].astype(np.float32),
+             px=sysd["protein_x"].astype(np.float32), pn=sysd["protein_n"].astype(np.float32),
+             pt=sysd["protein_type"], pid=sysd["protein_id"].astype(np.int32), bt=sysd["bond_type"],
+             bi=sysd["bond_i"].astype(np.int32), bj=sysd["bond_j"].astype(np.int32))
+    code = f"""
+import sys, json, ctypes as C, numpy as np
+sys.path.insert(0, {ROOT!r})
+from oracle import REF_SO
+L = C.CDLL(REF_SO)
+z = np.load({path!r})
+P = lambda a, t: np.ascontiguousarray(a).ctypes.data_as(C.POINTER(t))
+L.ref_run.restype = C.c_double
+nf = C.c_long()
+args = [len(z['lx']), P(z['lx'], C.c_float), P(z['ln'], C.c_float), len(z['px']), P(z['px'], C.c_float),
+        P(z['pn'], C.c_float), P(z['pt'], C.c_int), P(z['pid'], C.c_int), len(z['bt']), P(z['bt'], C.c_int),
+        P(z['bi'], C.c_int), P(z['bj'], C.c_int), C.c_double(0.22), C.c_double(0.01), C.c_double(0.01),
+        C.c_int(7), C.c_int({steps}), C.c_int({n_lloyd}), C.c_int({threads}), C.byref(nf)]
+t = L.ref_run(*args)
+print(json.dumps(dict(seconds=t, n_final=nf.value)))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    try:
+        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=timeout, env=env)
+        line = [l for l in out.stdout.splitlines() if l.startswith("{")][-1]
+        res = json.loads(line)
+    except Exception as ex:   # reported, never fatal
+        return dict(error=str(ex)[:200])
+    finally:
+        try:
+            os.remove(path)
+        except OSError:
+            pass
+    n = len(sysd["lipid_x"]) + len(sysd["protein_x"])
+    return dict(value=n * steps / res["seconds"], unit="particle-timesteps/s", cores=threads, kind="reference",
+                sample=f"OpenRBC reference main loop (fused pairwise + bonded + verlet_langevin + Voronoi "
+                       f"updates every 2 steps, FP32 as shipped) on the same {n}-particle system: {steps} steps "
+                       f"after a {n_lloyd}-iteration Lloyd init, {threads} OpenMP threads",
+                seconds=res["seconds"])
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--minimize", type=int, default=1000)
+    ap.add_argument("--skin", type=float, default=0.4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-lloyd", type=int, default=8)
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import openrbc_amd as orbc
+    t_build = time.perf_counter()
+    sysd = build_system(seed=7)
+    n = len(sysd["lipid_x"]) + len(sysd["protein_x"])
+    nb = len(sysd["bond_type"])
+    print(f"# system: {len(sysd['lipid_x'])} lipids + {len(sysd['protein_x'])} proteins = {n}, {nb} bonds "
+          f"({time.perf_counter() - t_build:.1f} s)", file=sys.stderr, flush=True)
+
+    # minimise once in FP64 (untimed; the reference always minimises first, src/openrbc.cpp:88-146)
+    sim64 = orbc.Simulation(precision=orbc.FP64, seed=7, skin=args.skin)
+    upload(sim64, sysd)
+    sim64.minimize(args.minimize)
+    state = download(sim64)
+    sim32 = orbc.Simulation(precision=orbc.MIXED, seed=7, skin=args.skin)
+    upload(sim32, sysd, state)
+    sim64.close()
+    sim64 = orbc.Simulation(precision=orbc.FP64, seed=7, skin=args.skin)
+    upload(sim64, sysd, state)
+
+    # force error of the mixed path against FP64 on the same configuration
+    ids = np.concatenate([sysd['lipid_id'], sysd['protein_id']]).astype(np.int64)
+    err_min = force_error(sim32, sim64, ids)
+    e_pairs = int(len(sim32.pairs()))
+    print(f"# minimised: {err_min} pairs={e_pairs}", file=sys.stderr, flush=True)
+
+    res = {}
+    for name, sim in (("mixed", sim32), ("fp64", sim64)):
+        if args.precision not in ("both", name):
+            continue
+        wall, tm, obs, rebuilds = timed_run(sim, args.steps, args.warmup)
+        kern_ms = tm["step_kernel_ms"] / max(1, tm["step_launches"])
+        B = algorithmic_bytes(name, n, e_pairs, nb)
+        res[name] = dict(value=n * args.steps / wall, ms_per_step=1e3 * wall / args.steps, kernel_ms=kern_ms,
+                         kernel_launches=tm["step_launches"], rebuild_ms=tm["rebuild_ms"], rebuilds=rebuilds,
+                         bytes_per_step=B, achieved_GBs=B / (kern_ms * 1e-3) / 1e9,
+                         temperature=obs["temperature"], potential_per_particle=obs["potential"] / n)
+        print(f"# {name}: {res[name]['value']:.4g} particle-steps/s, {res[name]['ms_per_step']:.3f} ms/step, "
+              f"kernel {kern_ms:.3f} ms, rebuilds {rebuilds} ({tm['rebuild_ms']:.1f} ms), "
+              f"T={obs['temperature']:.4f}", file=sys.stderr, flush=True)
+
+    # error again on the configuration reached by the mixed dynamics
+    err_dyn = None
+    if "mixed" in res and "fp64" in res:
+        upload(sim64, sysd, download(sim32))
+        err_dyn = force_error(sim32, sim64, ids)
+        print(f"# after dynamics: {err_dyn}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sysd, args.cpu_steps, args.cpu_lloyd, host_threads())
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    head = res.get("mixed") or res.get("fp64")
+    hname = "mixed" if "mixed" in res else "fp64"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_step_{hname}_c2.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "particle-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 forces / fp64 state" if hname == "mixed" else "fp64",
+        "data": "synthetic (OpenRBC example-large mesh populated by init_rbc rules, seed 7; Maxwell velocities)",
+        "config": {
+            "workload": "full-size RBC, example-large mesh: 2,639,690 lipids + 565,816 proteins, 584,066 bonds; "
+                        "Langevin kT=0.22 dt=0.01 eta=0.01; skin rebuild",
+            "n_particles": n, "n_bonds": nb, "pairs_within_cutoff": e_pairs, "skin": args.skin,
+            "precision": hname,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": head["achieved_GBs"],
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": head["achieved_GBs"] * 1e9 / HBM_PEAK,
+            "traffic": traffic,
+            "kernel": "k_step_langevin (fused pair + bond + Langevin)",
+            "kernel_ms": head["kernel_ms"],
+            "algorithmic_bytes_per_launch": head["bytes_per_step"],
+        },
+        "cpu_baseline": cpu,
+        "force_error": {"minimised": err_min, "after_dynamics": err_dyn},
+    }
+    if "mixed" in res and "fp64" in res:
+        line["fp64"] = {k: res["fp64"][k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_GBs", "rebuilds")}
+        line["fp64"]["roofline_frac"] = res["fp64"]["achieved_GBs"] * 1e9 / HBM_PEAK
+        line["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

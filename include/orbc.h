@@ -181,6 +181,13 @@ int orbc_within(orbc_ctx *ctx, long long npts, const float *pts, long long nq, c
 
 int orbc_synchronize(orbc_ctx *ctx);
 
+/* Kernel timing of orbc_run with HIP events on the context's stream:
+ * accumulated time of the fused step kernel and of neighbour rebuilds
+ * (measurement hook; the reference's Timers, src/timer.h:52-69). */
+int orbc_set_timing(orbc_ctx *ctx, int on);
+int orbc_get_timing(orbc_ctx *ctx, double *step_kernel_ms, long long *step_launches, double *rebuild_ms,
+                    long long *rebuilds);
+
 /* ---- initialiser: init_rbc (src/init_rbc.h:30-337) on the host ----------
  * Builds glycophorin, actin, band-3, spectrin and lipids from a triangle mesh
  * (vert: nv*3, edge: ne*2, face: nf*3, 0-based, the example-large text

@@ -121,6 +121,8 @@ SIGNATURES = [
     ("orbc_nearest", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, _i]),
     ("orbc_within", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, C.c_float, C.c_int, _i, _i]),
     ("orbc_synchronize", C.c_int, [_vp]),
+    ("orbc_set_timing", C.c_int, [_vp, C.c_int]),
+    ("orbc_get_timing", C.c_int, [_vp, _d, C.POINTER(C.c_longlong), _d, C.POINTER(C.c_longlong)]),
     ("orbc_init_rbc", C.c_int, [_d, C.c_longlong, _i, C.c_longlong, _i, C.c_longlong, C.c_double, C.c_ulonglong,
                                 C.POINTER(_System)]),
     ("orbc_system_free", None, [C.POINTER(_System)]),
@@ -310,6 +312,15 @@ class Simulation:
 
     def synchronize(self):
         self._check(self.L.orbc_synchronize(self.h))
+
+    def set_timing(self, on: bool = True):
+        self._check(self.L.orbc_set_timing(self.h, int(on)))
+
+    def timing(self) -> dict:
+        a, c = C.c_double(), C.c_double()
+        b, d = C.c_longlong(), C.c_longlong()
+        self._check(self.L.orbc_get_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return dict(step_kernel_ms=a.value, step_launches=b.value, rebuild_ms=c.value, rebuilds=d.value)
 
     # -- Voronoi cells ----------------------------------------------------------
     def voronoi_init(self, n_cells: int = 0, n_iter: int = 64):

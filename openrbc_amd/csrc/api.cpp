@@ -214,6 +214,7 @@ void commit(orbc_ctx *c) {
     c->nl = c->sp[0].n;
     c->np = c->sp[1].n;
     c->n = c->nl + c->np;
+    c->types_present = 0;
     if (c->n <= 0) throw std::runtime_error("no particles uploaded");
     if (c->n >= (1LL << 30)) throw std::runtime_error("more than 2^30 particles");
     c->stride = (c->n + 63) / 64 * 64;
@@ -235,6 +236,7 @@ void commit(orbc_ctx *c) {
             const int t = s == 0 ? 0 : S.type[u];
             if (t < 0 || t >= NT || (s == 1 && t == 0)) throw std::runtime_error("protein type must be 1..5");
             ht[k] = (uint8_t)t;
+            c->types_present |= 1u << t;
             hid[k] = S.id[u];
             hslot[k] = (uint32_t)k;
         }
@@ -274,7 +276,8 @@ void set_cutoffs(orbc_ctx *c, double cs) {
         for (int b = 0; b < NT; ++b) {
             const double rc = ff_max_cut(a, b);
             cut2[a * NT + b] = rc > 0 ? (float)((rc + skin) * (rc + skin)) : -1.0f;
-            if (rc > 0) rmax = std::max(rmax, rc + skin);
+            // search reach only towards types present in the system
+            if (rc > 0 && (c->types_present >> b & 1u)) rmax = std::max(rmax, rc + skin);
         }
         rings[a] = std::max(1, (int)std::ceil(rmax / cs - 1e-12));
     }
@@ -374,6 +377,8 @@ void rebuild(orbc_ctx *c) {
     launch_refresh(n, c->x[c->cur], c->nrm[c->cur], c->type, ag, c->xq[c->cur], c->nq[c->cur], c->rb, c->anc, s);
     // 7. cell ranges and the Verlet list
     launch_cell_ranges(c->x[c->cur], n, g, c->cstart, c->cend, s);
+    // occupied cells (the sort buffers are free again after the permutation)
+    const long long nocc = launch_cell_list(c->x[c->cur], n, g, c->keys_alt, c->vals_alt, c->keys, c->sort_temp, s);
     for (int attempt = 0; attempt < 3; ++attempt) {
         if (c->kmax == 0) {
             c->kmax = 48;
@@ -381,6 +386,7 @@ void rebuild(orbc_ctx *c) {
         }
         NlistArgs a{};
         a.n = n; a.stride = c->stride; a.x = c->x[c->cur]; a.type = c->type; a.g = g;
+        a.xq = c->xq[c->cur]; a.anc = ag;
         for (int d = 0; d < 3; ++d) { a.box.lo[d] = c->cfg.box_lo[d]; a.box.hi[d] = c->cfg.box_hi[d];
                                       a.box.L[d] = c->cfg.box_hi[d] - c->cfg.box_lo[d]; a.box.per[d] = 0; }
         a.cstart = c->cstart; a.cend = c->cend; a.nbr = c->nbr; a.nnbr = c->nnbr; a.kmax = c->kmax;
@@ -389,7 +395,7 @@ void rebuild(orbc_ctx *c) {
             a.aff = c->voro.aff; a.centroids = c->voro.c;
         }
         zero_flag(c, 2);
-        launch_nlist(a, s);
+        launch_nlist(a, c->keys, nocc, s);
         const unsigned mc = read_flag(c, 2);
         if ((int)mc <= c->kmax) break;
         c->mem.release(c->nbr);
@@ -400,6 +406,8 @@ void rebuild(orbc_ctx *c) {
     if (c->nbonds > 0)
         launch_bondlist(n, c->stride, c->slot, c->inv, c->bstart, c->bpartner, c->btype_d, c->bnd, c->nbnd, s);
     zero_flag(c, 0);
+    ORBC_HIP_CHECK(hipMemsetAsync(c->flags + 4, 0, 3 * sizeof(unsigned), s));
+    ORBC_HIP_CHECK(hipMemsetAsync(c->iflags, 0, sizeof(int), s));
     c->need_rebuild = false;
     c->moved = false;
     c->since_build = 0;
@@ -498,6 +506,7 @@ int orbc_destroy(orbc_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     voronoi_free(c);
     c->mem.free_all();
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_d) (void)hipHostFree(c->pinned_d);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -739,37 +748,84 @@ static StepArgs step_args(orbc_ctx *c) {
     langevin_coeffs(c->cfg, a.gamma, a.sigma);
     a.seed = c->cfg.seed;
     a.step = (unsigned long long)c->step;
-    a.disp_max = c->flags;
+    a.dslot = c->flags + 4;
+    a.half_skin2 = (float)(0.25 * c->cfg.skin * c->cfg.skin);
+    a.stop = c->iflags;
+    a.done = c->iflags + 1;
     a.bad = c->flags + 1;
-    a.skip = nullptr;
-    a.done = nullptr;
     a.swizzle = 1;
     return a;
+}
+
+// time a kernel sequence with HIP events on the context stream
+static void ev_pair(orbc_ctx *c, size_t k) {
+    while (c->ev.size() < 2 * (k + 1)) {
+        hipEvent_t e;
+        ORBC_HIP_CHECK(hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+}
+
+static void run_langevin(orbc_ctx *c, long long nsteps) {
+    long long remaining = nsteps;
+    while (remaining > 0) {
+        long long m = std::min<long long>(c->batch, remaining);
+        if (c->cfg.rebuild_period > 0) m = std::min<long long>(m, std::max<long long>(1, c->cfg.rebuild_period - c->since_build));
+        ORBC_HIP_CHECK(hipMemsetAsync(c->iflags + 1, 0, sizeof(int), c->stream));
+        const int cur0 = c->cur;
+        if (c->timing) ev_pair(c, (size_t)m);
+        for (long long s = 0; s < m; ++s) {
+            StepArgs a = step_args(c);
+            a.step = (unsigned long long)(c->step + s);
+            if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s], c->stream));
+            launch_step_langevin(a, c->mixed(), c->stream);
+            if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s + 1], c->stream));
+            c->cur = 1 - c->cur;
+        }
+        int fl[2];
+        ORBC_HIP_CHECK(hipMemcpyAsync(fl, c->iflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
+        unsigned bad = 0;
+        ORBC_HIP_CHECK(hipMemcpyAsync(&bad, c->flags + 1, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        const long long done = fl[1];
+        if (c->timing)
+            for (long long s = 0; s < done; ++s) {
+                float ms = 0;
+                ORBC_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]));
+                c->t_step_ms += ms;
+                c->n_step_launches++;
+            }
+        c->cur = (cur0 + (int)(done & 1)) & 1;
+        c->step += done;
+        c->since_build += done;
+        remaining -= done;
+        if (bad) throw std::runtime_error("non-finite state in orbc_run");
+        const bool period = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
+        if (fl[0] || period) {
+            if (c->timing) { ev_pair(c, 0); ORBC_HIP_CHECK(hipEventRecord(c->ev[0], c->stream)); }
+            rebuild(c);
+            if (c->timing) {
+                ORBC_HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+                ORBC_HIP_CHECK(hipEventSynchronize(c->ev[1]));
+                float ms = 0;
+                ORBC_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                c->t_rebuild_ms += ms;
+                c->n_timed_rebuilds++;
+            }
+        }
+    }
+    c->moved = true;
 }
 
 int orbc_run(orbc_ctx *c, long long nsteps) {
     return guarded(c, [&] {
         if (nsteps < 0) return fail(c, ORBC_ERR_INVALID, "nsteps < 0");
         ensure_list(c);
-        const double half2 = 0.25 * c->cfg.skin * c->cfg.skin;
         if (c->cfg.thermostat == ORBC_RUN_LANGEVIN) {
-            for (long long s = 0; s < nsteps; ++s) {
-                StepArgs a = step_args(c);
-                launch_step_langevin(a, c->mixed(), c->stream);
-                c->cur = 1 - c->cur;
-                c->step++;
-                c->since_build++;
-                // skin criterion (or fixed period), checked after every step
-                bool rb = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
-                if (!rb) {
-                    const unsigned bits = read_flag(c, 0);
-                    float d2;
-                    std::memcpy(&d2, &bits, 4);
-                    rb = !(d2 <= half2);
-                    if (read_flag(c, 1)) return fail(c, ORBC_ERR_NAN, "non-finite state in orbc_run");
-                }
-                if (rb && s + 1 < nsteps) rebuild(c);
-                else if (rb) c->need_rebuild = true;
+            try {
+                run_langevin(c, nsteps);
+            } catch (const std::runtime_error &e) {
+                return fail(c, ORBC_ERR_NAN, e.what());
             }
         } else {
             // Nose-Hoover, src/openrbc.cpp:176-243 with FUSED_INTEGRATOR
@@ -901,6 +957,25 @@ int orbc_get_pairs(orbc_ctx *c, long long cap, unsigned *id_i, unsigned *id_j, l
         *count = (long long)tot;
         return (int)ORBC_OK;
     });
+}
+
+int orbc_set_timing(orbc_ctx *c, int on) {
+    return guarded(c, [&] {
+        c->timing = on != 0;
+        c->t_step_ms = c->t_rebuild_ms = 0;
+        c->n_step_launches = c->n_timed_rebuilds = 0;
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_get_timing(orbc_ctx *c, double *step_kernel_ms, long long *step_launches, double *rebuild_ms,
+                    long long *rebuilds) {
+    if (!c) return ORBC_ERR_INVALID;
+    if (step_kernel_ms) *step_kernel_ms = c->t_step_ms;
+    if (step_launches) *step_launches = c->n_step_launches;
+    if (rebuild_ms) *rebuild_ms = c->t_rebuild_ms;
+    if (rebuilds) *rebuilds = c->n_timed_rebuilds;
+    return ORBC_OK;
 }
 
 int orbc_synchronize(orbc_ctx *c) {

@@ -88,7 +88,7 @@ struct orbc_ctx {
     uint32_t *bstart = nullptr, *bpartner = nullptr;
     uint8_t *btype_d = nullptr;
     long long nbonds = 0;
-    unsigned *flags = nullptr;   // [0] disp_max bits [1] bad [2] max_count
+    unsigned *flags = nullptr;   // [0] disp_max bits [1] bad [2] max_count [4..6] step displacement slots
     int *iflags = nullptr;       // [0] skip [1] done
     double *red = nullptr;       // reduction partials
     double *scal = nullptr;      // [0] zeta [1..7] sums
@@ -105,6 +105,13 @@ struct orbc_ctx {
     bool moved = false;          // positions changed by a standalone kernel since the last check
     long long step = 0, rebuilds = 0, last_pairs = 0, since_build = 0;
     bool nh_started = false;
+    unsigned types_present = 0;  // bit t set if some particle has type t
+    int batch = 32;              // fused steps launched per host synchronisation
+    // kernel timing (orbc_set_timing)
+    bool timing = false;
+    double t_step_ms = 0, t_rebuild_ms = 0;
+    long long n_step_launches = 0, n_timed_rebuilds = 0;
+    std::vector<hipEvent_t> ev;
 
     // Voronoi (k_voronoi / voronoi.cpp)
     struct Voro {

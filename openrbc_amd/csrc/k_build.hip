@@ -15,6 +15,10 @@ namespace orbc {
 
 namespace {
 
+__device__ inline float normsq_unfused(float a, float b, float c) {
+    return __fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c));
+}
+
 __constant__ float c_list_cut2[36];   // (rc(a,b) + skin)^2, <= 0 if a,b never interact
 __constant__ int c_rings[NT];         // grid rings a type searches
 
@@ -163,71 +167,209 @@ __global__ __launch_bounds__(BLOCK) void k_cell_ranges(const double *x, long lon
     if (c != cn) cend[c] = (uint32_t)(i + 1);
 }
 
-__device__ inline float normsq_unfused(float a, float b, float c) {
-    return __fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c));
+// ---- Verlet list -------------------------------------------------------
+// Candidate test in the FP32 anchor geometry of the force kernel (a superset
+// test: r^2 < (rc + skin)^2), plus the Voronoi stencil filter when asked.
+struct NlCtx {
+    const NlistArgs &a;
+    long long i;
+    int ti;
+    float4 xi;
+    int ax, ay, az;
+    float c0, c1, c2;
+    int cnt;
+};
+
+__device__ inline bool nl_accept(const NlCtx &c, uint32_t j, float4 xj, int tj, const float *cut2tab) {
+    if (j == (uint32_t)c.i) return false;
+    const float cut2 = cut2tab[c.ti * NT + tj];
+    if (!(cut2 > 0.f)) return false;
+    const Anchor &g = c.a.anc;
+    const uint32_t w = __float_as_uint(xj.w);
+    const float dx = (c.xi.x - xj.x) + g.h * (float)(c.ax - (int)(w & g.mx));
+    const float dy = (c.xi.y - xj.y) + g.h * (float)(c.ay - (int)((w >> g.shy) & g.my));
+    const float dz = (c.xi.z - xj.z) + g.h * (float)(c.az - (int)(w >> g.shz));
+    if (!(dx * dx + dy * dy + dz * dz < cut2)) return false;
+    if (c.a.aff) {
+        // reference stencil radii: pp 9, pl 8, ll 6 (src/compute_pairwise_fused.h:141-284)
+        const int cb = c.a.aff[j];
+        const float d2 = normsq_unfused(__fsub_rn(c.a.centroids[3 * cb], c.c0), __fsub_rn(c.a.centroids[3 * cb + 1], c.c1),
+                                        __fsub_rn(c.a.centroids[3 * cb + 2], c.c2));
+        const int nl = (c.ti == 0) + (tj == 0);
+        const float Rs = nl == 2 ? 6.0f : (nl == 1 ? 8.0f : 9.0f);
+        if (!(d2 < Rs * Rs)) return false;
+    }
+    return true;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a) {
-    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n) return;
-    const double xi = a.x[3 * i], yi = a.x[3 * i + 1], zi = a.x[3 * i + 2];
-    const int ti = a.type[i];
-    const int R = c_rings[ti];
-    const int cx = cell_coord(xi, a.g.lo[0], a.g.cs, a.g.dim[0]);
-    const int cy = cell_coord(yi, a.g.lo[1], a.g.cs, a.g.dim[1]);
-    const int cz = cell_coord(zi, a.g.lo[2], a.g.cs, a.g.dim[2]);
-    int ca = -1;
-    float c0 = 0, c1 = 0, c2 = 0;
-    if (a.aff) {
-        ca = a.aff[i];
-        c0 = a.centroids[3 * ca]; c1 = a.centroids[3 * ca + 1]; c2 = a.centroids[3 * ca + 2];
-    }
-    int cnt = 0;
-    for (int dz = -R; dz <= R; ++dz) {
-        int z = cz + dz;
-        if (a.box.per[2]) z = (z + a.g.dim[2]) % a.g.dim[2];
-        else if (z < 0 || z >= a.g.dim[2]) continue;
-        for (int dy = -R; dy <= R; ++dy) {
-            int y = cy + dy;
-            if (a.box.per[1]) y = (y + a.g.dim[1]) % a.g.dim[1];
-            else if (y < 0 || y >= a.g.dim[1]) continue;
-            for (int dx = -R; dx <= R; ++dx) {
-                int xx = cx + dx;
-                if (a.box.per[0]) xx = (xx + a.g.dim[0]) % a.g.dim[0];
-                else if (xx < 0 || xx >= a.g.dim[0]) continue;
-                const long long c = cell_lin(xx, y, z, a.g);
-                const uint32_t s = a.cstart[c], e = a.cend[c];
-                for (uint32_t j = s; j < e; ++j) {
-                    if (j == (uint32_t)i) continue;
-                    const int tj = a.type[j];
-                    const float cut2 = c_list_cut2[ti * NT + tj];
-                    if (cut2 <= 0.f) continue;
-                    double ddx = xi - a.x[3 * (long long)j];
-                    double ddy = yi - a.x[3 * (long long)j + 1];
-                    double ddz = zi - a.x[3 * (long long)j + 2];
-                    if (a.box.per[0]) ddx -= a.box.L[0] * rint(ddx / a.box.L[0]);
-                    if (a.box.per[1]) ddy -= a.box.L[1] * rint(ddy / a.box.L[1]);
-                    if (a.box.per[2]) ddz -= a.box.L[2] * rint(ddz / a.box.L[2]);
-                    const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (r2 >= (double)cut2) continue;
-                    if (a.aff) {
-                        // reference stencil radii: pp 9, pl 8, ll 6 (src/compute_pairwise_fused.h:141-284)
-                        const int cb = a.aff[j];
-                        const float d2 = normsq_unfused(__fsub_rn(a.centroids[3 * cb], c0),
-                                                        __fsub_rn(a.centroids[3 * cb + 1], c1),
-                                                        __fsub_rn(a.centroids[3 * cb + 2], c2));
-                        const int nl = (ti == 0) + (tj == 0);
-                        const float Rs = nl == 2 ? 6.0f : (nl == 1 ? 8.0f : 9.0f);
-                        if (!(d2 < Rs * Rs)) continue;
-                    }
-                    if (cnt < a.kmax) a.nbr[(long long)cnt * a.stride + i] = j;
-                    ++cnt;
-                }
+__device__ inline void nl_test(NlCtx &c, uint32_t j, float4 xj, int tj, const float *cut2tab) {
+    if (!nl_accept(c, j, xj, tj, cut2tab)) return;
+    if (c.cnt < c.a.kmax) c.a.nbr[(long long)c.cnt * c.a.stride + c.i] = j;
+    ++c.cnt;
+}
+
+// particle-centric scan of the cells at Chebyshev distance [rmin, rmax] from (cx,cy,cz)
+__device__ inline void nl_scan_cells(NlCtx &c, int cx, int cy, int cz, int rmin, int rmax, const float *cut2tab) {
+    const Grid &g = c.a.g;
+    for (int dz = -rmax; dz <= rmax; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.dim[2]) continue;
+        for (int dy = -rmax; dy <= rmax; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= g.dim[1]) continue;
+            for (int dx = -rmax; dx <= rmax; ++dx) {
+                const int x = cx + dx;
+                if (x < 0 || x >= g.dim[0]) continue;
+                if (max(abs(dx), max(abs(dy), abs(dz))) < rmin) continue;
+                const long long cl = cell_lin(x, y, z, g);
+                const uint32_t e = c.a.cend[cl];
+                for (uint32_t j = c.a.cstart[cl]; j < e; ++j) nl_test(c, j, c.a.xq[j], c.a.type[j], cut2tab);
             }
         }
     }
-    a.nnbr[i] = (uint32_t)(cnt < a.kmax ? cnt : a.kmax);
-    atomicMax(a.max_count, (unsigned)cnt);
+}
+
+constexpr int NL_CAP = 256;          // candidates staged per wave per pass
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline int wave_max_i(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// first particle of every occupied cell (particles are sorted by cell)
+__global__ __launch_bounds__(BLOCK) void k_cell_flags(const double *x, long long n, Grid g, uint32_t *flag) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    auto lin = [&](long long k) {
+        return cell_lin(cell_coord(x[3 * k], g.lo[0], g.cs, g.dim[0]), cell_coord(x[3 * k + 1], g.lo[1], g.cs, g.dim[1]),
+                        cell_coord(x[3 * k + 2], g.lo[2], g.cs, g.dim[2]), g);
+    };
+    flag[i] = (i == 0 || lin(i) != lin(i - 1)) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cell_compact(const uint32_t *flag, const uint32_t *pos, long long n,
+                                                        uint32_t *occ) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n && flag[i]) occ[pos[i]] = (uint32_t)i;
+}
+
+// One wavefront per occupied grid cell.  The particles of the cell's 3x3x3
+// neighbourhood are staged in LDS; the 64 lanes are split into (particle,
+// stripe) pairs -- P = cell particles rounded up to a power of two, S = 64/P
+// stripes -- and each round tests S consecutive candidates for every
+// particle.  Hits are appended in candidate order with a ballot-based
+// compaction, so the list order is deterministic.  Particle types whose
+// cutoff + skin exceeds one cell (band-3 / actin with the sigma-3.4 LJ) scan
+// the second ring of cells themselves afterwards.
+__global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *occ, long long nocc) {
+    __shared__ float4 s_x[BLOCK / 64][NL_CAP];
+    __shared__ uint32_t s_j[BLOCK / 64][NL_CAP];
+    __shared__ uint32_t s_pref[BLOCK / 64][28];
+    __shared__ uint32_t s_cst[BLOCK / 64][27];
+    __shared__ float s_cut2[36];
+    if (threadIdx.x < 36) s_cut2[threadIdx.x] = c_list_cut2[threadIdx.x];
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = __lane_id();
+    // XCD-aware remap: consecutive cells (whose list rows share cache lines)
+    // are built by blocks of the same XCD, so the lines complete in one L2
+    const unsigned nbk = gridDim.x, q = nbk >> 3, rr = nbk & 7u, xcd = blockIdx.x & 7u, kk = blockIdx.x >> 3;
+    const long long lb = (xcd < rr) ? (long long)xcd * (q + 1) + kk : (long long)rr * (q + 1) + (long long)(xcd - rr) * q + kk;
+    const long long cell = lb * (BLOCK / 64) + w;
+    if (cell >= nocc) return;                       // whole wave; no block barrier below
+    const Grid &g = a.g;
+    const long long p0 = occ[cell], p1 = (cell + 1 < nocc) ? occ[cell + 1] : a.n;
+    const int cx = cell_coord(a.x[3 * p0], g.lo[0], g.cs, g.dim[0]);
+    const int cy = cell_coord(a.x[3 * p0 + 1], g.lo[1], g.cs, g.dim[1]);
+    const int cz = cell_coord(a.x[3 * p0 + 2], g.lo[2], g.cs, g.dim[2]);
+    // ranges of the 27 neighbour cells, prefix counts
+    uint32_t st = 0, cntc = 0;
+    if (lane < 27) {
+        const int xx = cx + lane % 3 - 1, yy = cy + (lane / 3) % 3 - 1, zz = cz + lane / 9 - 1;
+        if (xx >= 0 && yy >= 0 && zz >= 0 && xx < g.dim[0] && yy < g.dim[1] && zz < g.dim[2]) {
+            const long long cl = cell_lin(xx, yy, zz, g);
+            st = a.cstart[cl];
+            cntc = a.cend[cl] - st;
+        }
+    }
+    uint32_t inc = cntc;
+    for (int off = 1; off < 32; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    const uint32_t total = __shfl(inc, 26, 64);
+    if (lane < 27) { s_pref[w][lane + 1] = inc; s_cst[w][lane] = st; }
+    if (lane == 0) s_pref[w][0] = 0;
+    wave_sync();
+    const long long np = p1 - p0;
+    for (long long pc = 0; pc < np; pc += 64) {
+        const int npc = (int)min(64LL, np - pc);
+        int P = 1;
+        while (P < npc) P <<= 1;
+        const int S = 64 / P;
+        const int pslot = lane / S, stripe = lane % S;
+        const bool act = pslot < npc;
+        const long long i = p0 + pc + (act ? pslot : 0);
+        NlCtx c{a, i, 0, make_float4(0, 0, 0, 0), 0, 0, 0, 0.f, 0.f, 0.f, 0};
+        if (act) {
+            c.ti = a.type[i];
+            c.xi = a.xq[i];
+            const uint32_t wi = __float_as_uint(c.xi.w);
+            c.ax = (int)(wi & a.anc.mx); c.ay = (int)((wi >> a.anc.shy) & a.anc.my); c.az = (int)(wi >> a.anc.shz);
+            if (a.aff) {
+                const int ca = a.aff[i];
+                c.c0 = a.centroids[3 * ca]; c.c1 = a.centroids[3 * ca + 1]; c.c2 = a.centroids[3 * ca + 2];
+            }
+        }
+        const unsigned long long gmask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << (pslot * S));
+        const unsigned long long below = (stripe == 0) ? 0ull : (((1ull << stripe) - 1ull) << (pslot * S));
+        int cnt = 0;
+        for (uint32_t off = 0; off < total; off += NL_CAP) {
+            const uint32_t m = min((uint32_t)NL_CAP, total - off);
+            wave_sync();
+            for (uint32_t t = lane; t < m; t += 64) {
+                const uint32_t gidx = off + t;
+                int lo = 0, hi = 27;                // largest k with pref[k] <= gidx
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_pref[w][mid] <= gidx) lo = mid; else hi = mid;
+                }
+                const uint32_t j = s_cst[w][lo] + (gidx - s_pref[w][lo]);
+                s_x[w][t] = a.xq[j];
+                s_j[w][t] = j | ((uint32_t)a.type[j] << 29);
+            }
+            wave_sync();
+            for (uint32_t t0 = 0; t0 < m; t0 += S) {
+                const uint32_t t = t0 + stripe;
+                bool hit = false;
+                uint32_t j = 0;
+                if (act && t < m) {
+                    const uint32_t jt = s_j[w][t];
+                    j = jt & 0x1fffffffu;
+                    hit = nl_accept(c, j, s_x[w][t], (int)(jt >> 29), s_cut2);
+                }
+                const unsigned long long hm = __ballot(hit);
+                if (hit) {
+                    const int pos = cnt + __popcll(hm & below);
+                    if (pos < a.kmax) a.nbr[(long long)pos * a.stride + i] = j;
+                }
+                cnt += __popcll(hm & gmask);
+            }
+        }
+        if (act && stripe == 0) {
+            c.cnt = cnt;
+            if (c_rings[c.ti] >= 2) nl_scan_cells(c, cx, cy, cz, 2, c_rings[c.ti], s_cut2);
+            a.nnbr[i] = (uint32_t)(c.cnt < a.kmax ? c.cnt : a.kmax);
+            cnt = c.cnt;
+        }
+        const int mc = wave_max_i((act && stripe == 0) ? cnt : 0);
+        if (lane == 0 && mc > a.kmax) atomicMax(a.max_count, (unsigned)mc);
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bondlist(long long n, long long stride, const uint32_t *slot,
@@ -259,7 +401,9 @@ __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, co
         if (!isfinite(d2)) d2 = 3.0e38f;
     }
     for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off, 64));
-    if (__lane_id() == 0 && d2 > 0.f) atomicMax(disp_max, __float_as_uint(d2));
+    // read before the atomic: only waves that raise the maximum write
+    if (__lane_id() == 0 && d2 > 0.f && __float_as_uint(d2) > __hip_atomic_load(disp_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(disp_max, __float_as_uint(d2));
 }
 
 inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -315,9 +459,23 @@ void launch_cell_ranges(const double *x, long long n, Grid g, uint32_t *cstart, 
     hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(n)), dim3(BLOCK), 0, s, x, n, g, cstart, cend);
 }
 
-void launch_nlist(const NlistArgs &a, hipStream_t s) {
-    if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_nlist, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a);
+long long launch_cell_list(const double *x, long long n, Grid g, uint32_t *flag, uint32_t *pos, uint32_t *occ,
+                           void *scan_temp, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_cell_flags, dim3(nblk(n)), dim3(BLOCK), 0, s, x, n, g, flag);
+    exclusive_scan_u32(flag, pos, n, scan_temp, s);
+    hipLaunchKernelGGL(k_cell_compact, dim3(nblk(n)), dim3(BLOCK), 0, s, flag, pos, n, occ);
+    uint32_t last[2];
+    ORBC_HIP_CHECK(hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, s));
+    ORBC_HIP_CHECK(hipMemcpyAsync(&last[1], flag + n - 1, 4, hipMemcpyDeviceToHost, s));
+    ORBC_HIP_CHECK(hipStreamSynchronize(s));
+    return (long long)last[0] + last[1];
+}
+
+void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, hipStream_t s) {
+    if (a.n <= 0 || nocc <= 0) return;
+    const long long nb = (nocc + BLOCK / 64 - 1) / (BLOCK / 64);
+    hipLaunchKernelGGL(k_nlist, dim3((unsigned)nb), dim3(BLOCK), 0, s, a, occ, nocc);
 }
 
 void launch_bondlist(long long n, long long stride, const uint32_t *slot, const uint32_t *inv,

@@ -19,6 +19,8 @@
 // whose positions are offsets from an integer anchor grid, so dx is exact to
 // FP32 rounding of O(1) numbers; FP64 evaluates everything in FP64.  The
 // integrator update and all reductions are FP64 in both.
+#include <type_traits>
+
 #include "orbc_internal.h"
 
 namespace orbc {
@@ -50,30 +52,60 @@ __device__ inline double add_rn(double a, double b) { return __dadd_rn(a, b); }
 __device__ inline float r2_rn(float x, float y, float z) { return __fmaf_rn(z, z, __fmaf_rn(y, y, __fmul_rn(x, x))); }
 __device__ inline double r2_rn(double x, double y, double z) { return __fma_rn(z, z, __fma_rn(y, y, __dmul_rn(x, x))); }
 
+// Force-field table staged in LDS by every force kernel: all pair types go
+// through the same code path (lipid-lipid and lipid-protein are both the
+// anisotropic form with different parameters), so a wave does not diverge on
+// the pair type except for the rare LJ / protein-protein terms.
+template <typename R>
+struct FFS {
+    R cut2p[36], cutp[36], rep[36], att[36], alpha[36], cut2lj[36], lj1[36], lj2[36];
+    int aniso[36];
+    R K[NBT], r0[NBT];
+};
+
+template <typename R>
+__device__ inline void load_ffs(FFS<R> *s) {
+    const FFTab<R> &t = ff<R>();
+    for (int k = threadIdx.x; k < 36; k += blockDim.x) {
+        s->cut2p[k] = t.cut2_poly[k]; s->cutp[k] = t.cut_poly[k]; s->rep[k] = t.rep[k]; s->att[k] = t.att[k];
+        s->alpha[k] = t.alpha[k]; s->cut2lj[k] = t.cut2_lj[k]; s->lj1[k] = t.lj1[k]; s->lj2[k] = t.lj2[k];
+        s->aniso[k] = t.aniso[k];
+    }
+    if (threadIdx.x < NBT) { s->K[threadIdx.x] = t.bond_K[threadIdx.x]; s->r0[threadIdx.x] = t.bond_r0[threadIdx.x]; }
+    __syncthreads();
+}
+
+// |d| from r2 = |d|^2: rinv by the hardware reciprocal square root, then one
+// Newton step on r so that cut - r (raised to the 7th power in the repulsion)
+// carries the rounding of r2 only.
+__device__ inline void radius(float r2, float &r, float &rinv) {
+    rinv = rsq(r2);
+    r = r2 * rinv;
+    r = __fmaf_rn(0.5f * __fmaf_rn(-r, r, r2), rinv, r);
+}
+__device__ inline void radius(double r2, double &r, double &rinv) {
+    rinv = rsq(r2);
+    r = r2 * rinv;
+}
+
 // One directed pair (owner i, neighbour j, d = x_i - x_j).  Returns true if
-// the pair interacts (same test as the oracle, bit for bit).
+// the pair interacts (same selection test as the oracle, bit for bit).
 template <typename R, bool ENERGY>
-__device__ __forceinline__ bool pair_ij(int ti, int tj, R dx, R dy, R dz, R r2, R nix, R niy, R niz, R njx,
-                                        R njy, R njz, Acc<R> &acc) {
+__device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R dy, R dz, R r2, R nix, R niy,
+                                        R niz, R njx, R njy, R njz, Acc<R> &acc) {
     if (!(r2 > R(1e-5))) return false;
     const int k = ti * NT + tj;
-    R cut2p, cutp, rep, att, alpha;
-    int aniso;
-    if (k == 0) {
-        cut2p = R(kCut2LL); cutp = R(kCutLL); rep = R(kRepLL); att = R(kAttLL); alpha = R(kAlphaLL); aniso = 1;
-    } else {
-        const FFTab<R> &t = ff<R>();
-        cut2p = t.cut2_poly[k]; cutp = t.cut_poly[k]; rep = t.rep[k]; att = t.att[k]; alpha = t.alpha[k];
-        aniso = t.aniso[k];
-    }
     R fx, fy, fz;
-    if (r2 < cut2p) {
-        const R rinv = rsq(r2);
-        const R r = r2 * rinv;
+    if (r2 < t.cut2p[k]) {
+        R r, rinv;
+        radius(r2, r, rinv);
         const R ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
-        const R s = cutp - r;
-        const R s3 = s * s * s, s4 = s * s3, s7 = s3 * s4;
-        if (aniso) {
+        const R s = t.cutp[k] - r;
+        const R s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+        const R rep = t.rep[k], att = t.att[k];
+        if (t.aniso[k]) {
+            // src/pairwise_kernel.h:37-62
+            const R alpha = t.alpha[k];
             const R ninj = nix * njx + niy * njy + niz * njz;
             const R nie = nix * ex + niy * ey + niz * ez;
             const R nje = njx * ex + njy * ey + njz * ez;
@@ -84,22 +116,23 @@ __device__ __forceinline__ bool pair_ij(int ti, int tj, R dx, R dy, R dz, R r2, 
             const R ua = att * s4;
             const R aua = alpha * ua;
             const R auar = aua * rinv;
-            const R fra = R(8) * rep * s7 + A * R(4) * att * s3;
+            const R fra = s3 * (R(8) * rep * s4 + R(4) * A * att);
             fx = fra * ex + auar * (nje * pix + nie * pjx);
             fy = fra * ey + auar * (nje * piy + nie * pjy);
             fz = fra * ez + auar * (nje * piz + nie * pjz);
             acc.tx -= aua * pjx;
             acc.ty -= aua * pjy;
             acc.tz -= aua * pjz;
-            if (ENERGY) acc.e += R(0.5) * (rep * s4 * s4 + A * ua);
+            if (ENERGY) acc.e += R(0.5) * s4 * (rep * s4 + A * att);
         } else {
-            const R fr = R(8) * rep * s7;
+            // src/pairwise_kernel_fused.h:79-97
+            const R fr = R(8) * rep * s3 * s4;
             fx = fr * ex; fy = fr * ey; fz = fr * ez;
             if (ENERGY) acc.e += R(0.5) * rep * s4 * s4;
         }
     } else {
-        const FFTab<R> &t = ff<R>();
-        if (!(r2 < t.cut2_lj[k])) return false;
+        if (!(r2 < t.cut2lj[k])) return false;
+        // src/pairwise_kernel_fused.h:63-77 (WCA, unshifted)
         const R r2inv = R(1) / r2;
         const R r6inv = r2inv * r2inv * r2inv;
         const R fpair = r6inv * (t.lj1[k] * r6inv - t.lj2[k]) * r2inv;
@@ -111,17 +144,17 @@ __device__ __forceinline__ bool pair_ij(int ti, int tj, R dx, R dy, R dz, R r2, 
     return true;
 }
 
-// Bond on its owner i: d = x_j - x_i.
+// Bond on its owner i: d = x_j - x_i, F_i = K (1 - r0/r) d (src/compute_bonded.h:112-115).
 template <typename R, bool ENERGY>
-__device__ __forceinline__ void bond_ij(int bt, R dx, R dy, R dz, Acc<R> &acc) {
-    const FFTab<R> &t = ff<R>();
+__device__ __forceinline__ void bond_ij(const FFS<R> &t, int bt, R dx, R dy, R dz, Acc<R> &acc) {
     const R r2 = dx * dx + dy * dy + dz * dz;
-    const R rinv = rsq(r2);
-    const R K = t.bond_K[bt], r0 = t.bond_r0[bt];
-    const R c = K * (R(1) - r0 * rinv);
+    R r, rinv;
+    radius(r2, r, rinv);
+    const R K = t.K[bt], r0 = t.r0[bt];
+    const R c = K * ((r - r0) * rinv);
     acc.fx += c * dx; acc.fy += c * dy; acc.fz += c * dz;
     if (ENERGY) {
-        const R dr = r2 * rinv - r0;
+        const R dr = r - r0;
         acc.e += R(0.25) * K * dr * dr;
         acc.w += R(-0.5) * c * r2;
     }
@@ -143,33 +176,52 @@ __device__ inline AnchorI unpack(uint32_t w, const Anchor &a) {
     return AnchorI{(int)(w & a.mx), (int)((w >> a.shy) & a.my), (int)(w >> a.shz)};
 }
 
-// ---- per-particle force accumulation (both precisions) ------------------
-// Mixed: i's own mirror values are passed in; FP64 reads x, n directly.
+template <typename R>
+__device__ inline void add_to(Acc<double> &o, const Acc<R> &a) {
+    o.fx += (double)a.fx; o.fy += (double)a.fy; o.fz += (double)a.fz;
+    o.tx += (double)a.tx; o.ty += (double)a.ty; o.tz += (double)a.tz;
+    o.e += (double)a.e; o.w += (double)a.w;
+}
+
+constexpr int UNROLL = 4;   // neighbours gathered per group (memory-level parallelism)
+
+// ---- per-particle force accumulation ---------------------------------------
+// Mixed: FP32 pair terms from the FP32 mirrors, summed in FP32 over each group
+// of UNROLL neighbours and in FP64 across groups.
 template <bool ENERGY>
 __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, const float4 *xq, const float4 *nq,
                                                  const uint32_t *nbr, const uint32_t *nnbr, const uint32_t *bnd,
                                                  const uint8_t *nbnd, const Anchor &ag, float4 xi, float4 ni,
-                                                 Acc<float> &acc, bool pair, bool bond) {
+                                                 const FFS<float> &t, Acc<double> &out, bool pair, bool bond) {
     const int ti = __float_as_int(ni.w);
     const AnchorI ai = unpack(__float_as_uint(xi.w), ag);
     if (pair) {
         const int cnt = (int)nnbr[i];
         const uint32_t *nb = nbr + i;
-        for (int k = 0; k < cnt; ++k) {
-            const uint32_t j = nb[(long long)k * stride];
-            const float4 xj = xq[j];
-            const float4 nj = nq[j];
-            const AnchorI aj = unpack(__float_as_uint(xj.w), ag);
-            const float dx = add_rn(sub_rn(xi.x, xj.x), ag.h * (float)(ai.x - aj.x));
-            const float dy = add_rn(sub_rn(xi.y, xj.y), ag.h * (float)(ai.y - aj.y));
-            const float dz = add_rn(sub_rn(xi.z, xj.z), ag.h * (float)(ai.z - aj.z));
-            const float r2 = r2_rn(dx, dy, dz);
-            pair_ij<float, ENERGY>(ti, __float_as_int(nj.w), dx, dy, dz, r2, ni.x, ni.y, ni.z, nj.x, nj.y, nj.z,
-                                   acc);
+        for (int k0 = 0; k0 < cnt; k0 += UNROLL) {
+            uint32_t jj[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) jj[u] = (k0 + u < cnt) ? nb[(long long)(k0 + u) * stride] : (uint32_t)i;
+            float4 xj[UNROLL], nj[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) { xj[u] = xq[jj[u]]; nj[u] = nq[jj[u]]; }
+            Acc<float> acc;
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const AnchorI aj = unpack(__float_as_uint(xj[u].w), ag);
+                const float dx = add_rn(sub_rn(xi.x, xj[u].x), ag.h * (float)(ai.x - aj.x));
+                const float dy = add_rn(sub_rn(xi.y, xj[u].y), ag.h * (float)(ai.y - aj.y));
+                const float dz = add_rn(sub_rn(xi.z, xj[u].z), ag.h * (float)(ai.z - aj.z));
+                const float r2 = r2_rn(dx, dy, dz);     // the self entry padding a group gives r2 = 0
+                pair_ij<float, ENERGY>(t, ti, __float_as_int(nj[u].w), dx, dy, dz, r2, ni.x, ni.y, ni.z, nj[u].x,
+                                       nj[u].y, nj[u].z, acc);
+            }
+            add_to(out, acc);
         }
     }
     if (bond && nbnd) {
         const int nb = nbnd[i];
+        Acc<float> acc;
         for (int k = 0; k < nb; ++k) {
             const uint32_t e = bnd[(long long)k * stride + i];
             const uint32_t j = e & BOND_INDEX_MASK;
@@ -178,30 +230,42 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
             const float dx = (xj.x - xi.x) + ag.h * (float)(aj.x - ai.x);
             const float dy = (xj.y - xi.y) + ag.h * (float)(aj.y - ai.y);
             const float dz = (xj.z - xi.z) + ag.h * (float)(aj.z - ai.z);
-            bond_ij<float, ENERGY>((int)(e >> 30), dx, dy, dz, acc);
+            bond_ij<float, ENERGY>(t, (int)(e >> 30), dx, dy, dz, acc);
         }
+        add_to(out, acc);
     }
 }
 
 template <bool ENERGY>
 __device__ __forceinline__ void accumulate_fp64(long long i, long long stride, const double *x, const double *nrm,
                                                 const uint8_t *type, const uint32_t *nbr, const uint32_t *nnbr,
-                                                const uint32_t *bnd, const uint8_t *nbnd, Acc<double> &acc, bool pair,
-                                                bool bond) {
+                                                const uint32_t *bnd, const uint8_t *nbnd, const FFS<double> &t,
+                                                Acc<double> &acc, bool pair, bool bond) {
     const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
     if (pair) {
         const double nix = nrm[3 * i], niy = nrm[3 * i + 1], niz = nrm[3 * i + 2];
         const int ti = type[i];
         const int cnt = (int)nnbr[i];
         const uint32_t *nb = nbr + i;
-        for (int k = 0; k < cnt; ++k) {
-            const long long j = nb[(long long)k * stride];
-            const double dx = sub_rn(xi, x[3 * j]);
-            const double dy = sub_rn(yi, x[3 * j + 1]);
-            const double dz = sub_rn(zi, x[3 * j + 2]);
-            const double r2 = r2_rn(dx, dy, dz);
-            pair_ij<double, ENERGY>(ti, type[j], dx, dy, dz, r2, nix, niy, niz, nrm[3 * j], nrm[3 * j + 1],
-                                    nrm[3 * j + 2], acc);
+        for (int k0 = 0; k0 < cnt; k0 += UNROLL) {
+            long long jj[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) jj[u] = (k0 + u < cnt) ? (long long)nb[(long long)(k0 + u) * stride] : i;
+            double xj[UNROLL][3], nj[UNROLL][3];
+            int tj[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                for (int d = 0; d < 3; ++d) { xj[u][d] = x[3 * jj[u] + d]; nj[u][d] = nrm[3 * jj[u] + d]; }
+                tj[u] = type[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const double dx = sub_rn(xi, xj[u][0]);
+                const double dy = sub_rn(yi, xj[u][1]);
+                const double dz = sub_rn(zi, xj[u][2]);
+                const double r2 = r2_rn(dx, dy, dz);
+                pair_ij<double, ENERGY>(t, ti, tj[u], dx, dy, dz, r2, nix, niy, niz, nj[u][0], nj[u][1], nj[u][2], acc);
+            }
         }
     }
     if (bond && nbnd) {
@@ -209,7 +273,7 @@ __device__ __forceinline__ void accumulate_fp64(long long i, long long stride, c
         for (int k = 0; k < nb; ++k) {
             const uint32_t e = bnd[(long long)k * stride + i];
             const long long j = e & BOND_INDEX_MASK;
-            bond_ij<double, ENERGY>((int)(e >> 30), x[3 * j] - xi, x[3 * j + 1] - yi, x[3 * j + 2] - zi, acc);
+            bond_ij<double, ENERGY>(t, (int)(e >> 30), x[3 * j] - xi, x[3 * j + 1] - yi, x[3 * j + 2] - zi, acc);
         }
     }
 }
@@ -217,22 +281,23 @@ __device__ __forceinline__ void accumulate_fp64(long long i, long long stride, c
 // ---- standalone force evaluation (compute_pairwise_fused / compute_bonded) --
 template <bool MIXED, bool ENERGY>
 __global__ __launch_bounds__(BLOCK) void k_forces(ForceArgs a, int pair, int bond) {
+    __shared__ FFS<typename std::conditional<MIXED, float, double>::type> t;
+    load_ffs(&t);
     const long long i = block_index(a.swizzle) * BLOCK + threadIdx.x;
     if (i >= a.n) return;
-    if (MIXED) {
-        Acc<float> acc;
+    Acc<double> acc;
+    if constexpr (MIXED) {
         accumulate_mixed<ENERGY>(i, a.stride, a.xq, a.nq, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, a.xq[i], a.nq[i],
-                                 acc, pair, bond);
-        a.f32[3 * i] += acc.fx; a.f32[3 * i + 1] += acc.fy; a.f32[3 * i + 2] += acc.fz;
-        a.t32[3 * i] += acc.tx; a.t32[3 * i + 1] += acc.ty; a.t32[3 * i + 2] += acc.tz;
-        if (ENERGY) { a.e[i] += acc.e; a.w[i] += acc.w; }
+                                 t, acc, pair, bond);
+        a.f32[3 * i] += (float)acc.fx; a.f32[3 * i + 1] += (float)acc.fy; a.f32[3 * i + 2] += (float)acc.fz;
+        a.t32[3 * i] += (float)acc.tx; a.t32[3 * i + 1] += (float)acc.ty; a.t32[3 * i + 2] += (float)acc.tz;
     } else {
-        Acc<double> acc;
-        accumulate_fp64<ENERGY>(i, a.stride, a.x, a.nrm, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd, acc, pair, bond);
+        accumulate_fp64<ENERGY>(i, a.stride, a.x, a.nrm, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
+                                t, acc, pair, bond);
         a.f64[3 * i] += acc.fx; a.f64[3 * i + 1] += acc.fy; a.f64[3 * i + 2] += acc.fz;
         a.t64[3 * i] += acc.tx; a.t64[3 * i + 1] += acc.ty; a.t64[3 * i + 2] += acc.tz;
-        if (ENERGY) { a.e[i] += acc.e; a.w[i] += acc.w; }
     }
+    if (ENERGY) { a.e[i] += acc.e; a.w[i] += acc.w; }
 }
 
 template <bool MIXED>
@@ -268,6 +333,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_export(ForceArgs a, unsigned *i
             const unsigned long long slot = atomicAdd(count, 1ull);
             if ((long long)slot < cap) { ids_i[slot] = idi; ids_j[slot] = idj; }
         }
+
     }
 }
 
@@ -284,45 +350,47 @@ __device__ inline void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, u
     }
 }
 
-__device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 // ---- fused timestep: forces + bonds + verlet_langevin ---------------------
 // src/openrbc.cpp:219-233 (compute_pairwise_fused, compute_bonded,
 // integrate(verlet_langevin)) in one pass; the Langevin update restates
 // src/integrate_langevin.h:275-297 in FP64 with counter-based noise.
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
-    if (a.skip && *a.skip) return;
+    const int k3 = (int)(a.step % 3ull);
+    // skin flag of this step's input positions; no block of this launch writes it
+    if (*a.stop || a.dslot[(k3 + 2) % 3] != 0u) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *a.stop = 1;
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.dslot[(k3 + 1) % 3] = 0u;
+        atomicAdd(a.done, 1);
+    }
+    __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
+    load_ffs(&tab);
     const long long i = block_index(a.swizzle) * BLOCK + threadIdx.x;
     const bool act = i < a.n;
     float d2 = 0.f;
     bool bad = false;
     if (act) {
         double fx, fy, fz, tx, ty, tz;
-        float4 xi4, ni4;
         int ti;
         uint32_t wi;
-        if (MIXED) {
-            xi4 = a.xq_in[i];
-            ni4 = a.nq_in[i];
-            Acc<float> acc;
+        Acc<double> acc;
+        if constexpr (MIXED) {
+            const float4 xi4 = a.xq_in[i];
+            const float4 ni4 = a.nq_in[i];
             accumulate_mixed<false>(i, a.stride, a.xq_in, a.nq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, xi4,
-                                    ni4, acc, true, a.has_bonds != 0);
-            fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
+                                    ni4, tab, acc, true, a.has_bonds != 0);
             ti = __float_as_int(ni4.w);
             wi = __float_as_uint(xi4.w);
         } else {
-            Acc<double> acc;
-            accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd, acc, true,
-                                   a.has_bonds != 0);
-            fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
+            accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
+                                   tab, acc, true, a.has_bonds != 0);
             ti = a.type[i];
             wi = a.anc[i];
         }
+        fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
         if (a.f_ext) { fx += a.f_ext[3 * i]; fy += a.f_ext[3 * i + 1]; fz += a.f_ext[3 * i + 2]; }
         // rotation: t = n x t; o += dt t; n = normalize(n + (o x n) dt)
         const double nx = a.n_in[3 * i], ny = a.n_in[3 * i + 1], nz = a.n_in[3 * i + 2];
@@ -333,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
         double gx = nx + (oy * nz - oz * ny) * dt;
         double gy = ny + (oz * nx - ox * nz) * dt;
         double gz = nz + (ox * ny - oy * nx) * dt;
-        const double gs = 1.0 / sqrt(gx * gx + gy * gy + gz * gz);
+        const double gs = rsqrt(gx * gx + gy * gy + gz * gz);
         gx *= gs; gy *= gs; gz *= gs;
         // translation with uniform [-1,1) noise (src/integrate_langevin.h:282-296)
         uint32_t c0 = a.id[i], c1 = (uint32_t)a.step, c2 = (uint32_t)(a.step >> 32), c3 = 0u;
@@ -367,10 +435,10 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
         bad = !(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(gx));
         if (bad) d2 = 3.0e38f;
     }
-    d2 = wave_max(d2);
-    if (__lane_id() == 0 && d2 > 0.f) atomicMax(a.disp_max, __float_as_uint(d2));
+    // one atomic per wave, and only when some lane moved past skin/2
+    const unsigned long long over = __ballot(d2 > a.half_skin2);
+    if (over && __lane_id() == (int)__ffsll((long long)over) - 1) atomicOr(a.dslot + k3, 1u);
     if (bad) atomicOr(a.bad, 1u);
-    if (a.done && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.done, 1);
 }
 
 // ---- fused forces + post_toque_final_update (NH second half) --------------
@@ -380,25 +448,25 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
 template <bool MIXED>
 __global__ __launch_bounds__(BLOCK) void k_step_nh_second(StepArgs a, double *f64, double *t64,
                                                           double *ke_partial, const double *zeta_p) {
+    __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
+    load_ffs(&tab);
     const long long i = block_index(a.swizzle) * BLOCK + threadIdx.x;
     const bool act = i < a.n;
     double ke = 0.0;
     if (act) {
         double fx, fy, fz, tx, ty, tz;
         int ti;
-        if (MIXED) {
-            Acc<float> acc;
+        Acc<double> acc;
+        if constexpr (MIXED) {
             accumulate_mixed<false>(i, a.stride, a.xq_in, a.nq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid,
-                                    a.xq_in[i], a.nq_in[i], acc, true, a.has_bonds != 0);
-            fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
+                                    a.xq_in[i], a.nq_in[i], tab, acc, true, a.has_bonds != 0);
             ti = __float_as_int(a.nq_in[i].w);
         } else {
-            Acc<double> acc;
-            accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd, acc, true,
-                                   a.has_bonds != 0);
-            fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
+            accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
+                                   tab, acc, true, a.has_bonds != 0);
             ti = a.type[i];
         }
+        fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
         if (a.f_ext) { fx += a.f_ext[3 * i]; fy += a.f_ext[3 * i + 1]; fz += a.f_ext[3 * i + 2]; }
         const double zeta = *zeta_p, dt = a.dt;
         const double nx = a.n_in[3 * i], ny = a.n_in[3 * i + 1], nz = a.n_in[3 * i + 2];

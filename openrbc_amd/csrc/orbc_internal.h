@@ -85,10 +85,16 @@ struct StepArgs {
     double dt;
     double gamma[NT], sigma[NT];
     unsigned long long seed, step;
-    unsigned *disp_max;        // float bits of max squared displacement since build
+    // skin criterion on the device: step k reads slot (k+2)%3 (displacement of
+    // its input positions), writes slot k%3 (of its output) and clears slot
+    // (k+1)%3 for step k+1.  A step whose input violates the skin, or that
+    // finds *stop set, does nothing and sets *stop, so the host can launch a
+    // batch of steps and rebuild once after it.
+    unsigned *dslot;           // 3 flags: some particle moved more than skin/2 since build
+    float half_skin2;          // (skin/2)^2
+    int *stop;                 // sticky: a step was skipped
+    int *done;                 // steps executed (incremented by block 0)
     unsigned *bad;             // nonzero if a non-finite value appeared
-    const int *skip;           // if nonnull and *skip: do nothing (graph replay guard)
-    int *done;                 // incremented once per executed step (block 0)
     int swizzle;               // XCD-aware block remap
 };
 
@@ -141,6 +147,8 @@ void launch_cell_ranges(const double *x, long long n, Grid g, uint32_t *cstart, 
 struct NlistArgs {
     long long n, stride;
     const double *x;
+    const float4 *xq;        // FP32 mirrors with anchors (refreshed just before)
+    Anchor anc;
     const uint8_t *type;
     Grid g;
     Box box;
@@ -152,7 +160,9 @@ struct NlistArgs {
     const int32_t *aff;      // Voronoi filter (null in exact mode)
     const float *centroids;
 };
-void launch_nlist(const NlistArgs &a, hipStream_t s);
+long long launch_cell_list(const double *x, long long n, Grid g, uint32_t *flag, uint32_t *pos, uint32_t *occ,
+                           void *scan_temp, hipStream_t s);
+void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, hipStream_t s);
 void set_list_cutoffs(const float *cut2_list, const int *rings, hipStream_t s);
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g,
                  unsigned *disp_max, hipStream_t s);

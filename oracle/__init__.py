@@ -41,6 +41,14 @@ def _c(a, dtype):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
+class _FF(C.Structure):
+    _fields_ = [(k, C.c_double * 36) for k in ("cut_poly", "cut2_poly", "rep", "att", "alpha")] + \
+               [("aniso", C.c_int * 36)] + \
+               [(k, C.c_double * 36) for k in ("cut2_lj", "lj1", "lj2")] + \
+               [("mass", C.c_double * 6), ("radius", C.c_double * 6), ("bond_r0", C.c_double * 4),
+                ("bond_K", C.c_double * 4)]
+
+
 class Oracle:
     """FP64 CPU restatement (oracle/orbc_oracle.c)."""
 
@@ -49,6 +57,8 @@ class Oracle:
             build(ref=False)
         L = C.CDLL(path)
         self.L = L
+        L.orc_ff_get.restype = C.POINTER(_FF)
+        L.orc_ff_get.argtypes = []
         L.orc_pair.restype = C.c_int
         L.orc_pair.argtypes = [C.c_int, C.c_int, _d, _d, _d, _d, _d, _d, _d]
         L.orc_pairs_brute_fp64.restype = C.c_int64
@@ -90,6 +100,15 @@ class Oracle:
         L.orc_pairs_voronoi_fp64.argtypes = [C.c_int64, _d, _i32, _i32, C.c_int64, _f, C.c_int64, _i64]
         L.orc_constrain_volume.restype = C.c_double
         L.orc_constrain_volume.argtypes = [C.c_int64, _d, _i32, _i32, C.c_int64, _f, C.c_double, C.c_double, _d]
+
+    def ff(self) -> dict:
+        """Force-field tables, 6x6 arrays indexed [owner type, neighbour type]."""
+        f = self.L.orc_ff_get().contents
+        out = {}
+        for name, _ in _FF._fields_:
+            a = np.array(getattr(f, name)[:])
+            out[name] = a.reshape(6, 6) if a.size == 36 else a
+        return out
 
     # --- pair sets ---------------------------------------------------------
     def _pairs(self, fn, *args):

@@ -1,0 +1,101 @@
+"""Write the golden fixtures of tests/golden/ from the OpenRBC reference itself.
+
+The reference (oracle/_ref/libref_openrbc.so, compiled in place from the
+reference sources by oracle/Makefile) is driven through oracle/ref_harness.cpp
+on small synthetic inputs; inputs and the reference's outputs are stored as
+.npz data.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+
+The fixtures are data only (inputs and expected outputs, FP32 as the shipped
+reference computes them); the tests compare the FP64 oracle against them.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle import Reference  # noqa: E402
+from _systems import patch, split  # noqa: E402
+
+
+def containers(s):
+    lip, prot = split(s)
+    bt, bi, bj = s["bonds"]
+    tag = s["id"].astype(np.int32)
+    return lip, prot, (bt.astype(np.int32), tag[bi], tag[bj])
+
+
+def main():
+    R = Reference()
+    out = {}
+    ff = R.ff_tables()
+    np.savez_compressed(os.path.join(HERE, "ff_tables.npz"), **ff)
+
+    # compute_pairwise_fused + compute_bonded on two membrane patches
+    for seed in (3, 8):
+        s = patch(12, 12, seed=seed)
+        lip, prot, bonds = containers(s)
+        lf, lt, pf, pt = R.forces(s["x"][lip], s["n"][lip], s["x"][prot], s["n"][prot], s["type"][prot],
+                                  s["id"][prot], bonds)
+        np.savez_compressed(os.path.join(HERE, f"forces_patch{seed}.npz"),
+                            lx=s["x"][lip].astype(np.float32), ln=s["n"][lip].astype(np.float32),
+                            px=s["x"][prot].astype(np.float32), pn=s["n"][prot].astype(np.float32),
+                            ptype=s["type"][prot].astype(np.int32), ptag=s["id"][prot].astype(np.int32),
+                            bt=bonds[0], bi=bonds[1], bj=bonds[2], lf=lf, lt=lt, pf=pf, pt=pt)
+
+    # verlet_langevin at kT = 0 (deterministic) and the two fused NH halves
+    s = patch(8, 8, seed=2)
+    lip, prot, _ = containers(s)
+    rng = np.random.default_rng(4)
+    n = len(s["type"])
+    f = rng.normal(0, 3, (n, 3)); t = rng.normal(0, 1, (n, 3))
+
+    def state(idx):
+        return {k: np.ascontiguousarray(a[idx], dtype=np.float32)
+                for k, a in (("x", s["x"]), ("v", s["v"]), ("n", s["n"]), ("o", s["o"]), ("f", f), ("t", t))}
+
+    L, P = state(lip), state(prot)
+    inp = {f"l{k}": v.copy() for k, v in L.items()}
+    inp.update({f"p{k}": v.copy() for k, v in P.items()})
+    ptype = s["type"][prot].astype(np.int32)
+    R.langevin(L, P, ptype, 0.0, 0.01, 0.01)
+    np.savez_compressed(os.path.join(HERE, "langevin_T0.npz"), ptype=ptype, **inp,
+                        **{f"out_l{k}": v for k, v in L.items()}, **{f"out_p{k}": v for k, v in P.items()})
+    for which in (0, 1):
+        L, P = state(lip), state(prot)
+        z = R.nh(which, L, P, ptype, 0.22, 0.01, 0.05, 7.0)
+        np.savez_compressed(os.path.join(HERE, f"nh{which}.npz"), ptype=ptype, zeta_in=0.05, Q=7.0, kT=0.22,
+                            zeta_out=z, **inp, **{f"out_l{k}": v for k, v in L.items()},
+                            **{f"out_p{k}": v for k, v in P.items()})
+
+    # temperature
+    T = R.temperature(s["v"][lip].astype(np.float32), s["v"][prot].astype(np.float32), ptype)
+    np.savez_compressed(os.path.join(HERE, "temperature.npz"), lv=s["v"][lip].astype(np.float32),
+                        pv=s["v"][prot].astype(np.float32), ptype=ptype, T=T)
+
+    # Morton keys and the reorder_morton permutation
+    pts = rng.uniform(-600, 600, (2000, 3)).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "morton.npz"), pts=pts, keys=R.morton(pts),
+                        reordered=R.reorder_morton(pts[:500]))
+
+    # k-d tree queries and Voronoi partition
+    cen = rng.uniform(-30, 30, (400, 3)).astype(np.float32)
+    q = rng.uniform(-35, 35, (1500, 3)).astype(np.float32)
+    near = R.kdtree_nearest(cen, q)
+    within = R.kdtree_within(cen, q[:200], 9.0)
+    counts = np.array([len(w) for w in within], np.int32)
+    flat = np.concatenate(within).astype(np.int32)
+    aff = R.partition(q, cen)
+    np.savez_compressed(os.path.join(HERE, "voronoi.npz"), centroids=cen, queries=q, nearest=near,
+                        within_counts=counts, within_flat=flat, partition=aff)
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -224,3 +224,27 @@ def test_langevin_run_tracks_oracle(oracle):
     assert np.abs(got["x"] - st["x"]).max() < 1e-9
     assert np.abs(got["n"] - st["n"]).max() < 1e-9
     assert sim.observables()["step"] == nsteps
+
+
+@pytest.mark.parametrize("precision", [orbc.FP64, orbc.MIXED])
+def test_run_with_skin_rebuilds(precision, oracle):
+    """A small skin forces rebuilds inside the batched run (device-side skin
+    check + sticky stop flag); the trajectory must still follow the oracle
+    stepping the exact pair set."""
+    s = patch(10, 10, seed=4)
+    seed = 5
+    sim, lip, prot = make_sim(s, precision, seed=seed, skin=0.03)
+    st = {k: s[k].copy() for k in ("x", "v", "n", "o")}
+    st.update(type=s["type"].astype(np.int32), id=s["id"].astype(np.uint32))
+    nsteps = 70
+    sim.run(nsteps)
+    for step in range(nsteps):
+        f, t, _, _ = oracle.forces(st["x"], st["n"], st["type"], bonds=s["bonds"])
+        st["f"], st["t"] = f, t
+        oracle.langevin(st, 0.22, 0.01, 0.01, seed, step)
+    got = gather(sim, lip, prot, "state")
+    tol = 1e-8 if precision == orbc.FP64 else 1e-3
+    assert np.abs(got["x"] - st["x"]).max() < tol
+    obs = sim.observables()
+    assert obs["step"] == nsteps
+    assert obs["rebuilds"] > 3
