@@ -3,13 +3,19 @@
 
 Workload: the full-size RBC of OpenRBC's example-large mesh (Evans-Fung
 biconcave, 9,128 vertices) populated by the init_rbc rules: 2,639,690 lipids +
-565,816 proteins = 3,205,506 particles, 584,066 bonds; Langevin at kT=0.22,
-dt=0.01, eta=0.01, seed 7, Verlet list rebuilt on the skin criterion.  The
+565,816 proteins = 3,205,506 particles, 584,066 bonds.  Langevin at kT=0.22,
+dt=0.01, eta=0.01, seed 7; Verlet list rebuilt on the skin criterion;
+Voronoi cells (n_lipid / 14, 64 Lloyd steps) updated every 2 steps and the
+volume constraint at the initial enclosed volume applied every step.  The
 mixed-precision step (FP32 pair/bond forces, FP64 state and reductions) is the
-headline ``value``; the plain FP64 drop-in is reported beside it with the
-force error of mixed vs FP64 on the same configuration.
+headline ``value``; the plain FP64 drop-in runs the same loop beside it, with
+the force error of mixed vs FP64 on the same configuration and the total-energy
+drift of both runs.  ``--mesh evans-fung`` generates the biconcave surface
+instead (frequency-34 geodesic triangulation, 11,562 vertices, 4,122,700
+particles).
 
-A "step" is one fused launch of forces + bonds + Langevin update over all
+A "step" is one pass of the loop: the Voronoi update (every second step), the
+volume push and one fused launch of forces + bonds + Langevin update over all
 particles (rebuilds, when the skin criterion fires, are inside the timed
 region).  Inputs are resident in HBM before timing starts.
 
@@ -41,9 +47,12 @@ def algorithmic_bytes(precision: str, n: int, e_pairs: int, n_bonds: int) -> flo
     return base * n + 8.0 * e_pairs + 12.0 * n_bonds
 
 
-def build_system(seed: int = 7):
+def build_system(mesh: str = "example-large", seed: int = 7):
     import openrbc_amd as orbc
-    v, e, f = orbc.load_example_large()
+    if mesh == "example-large":
+        v, e, f = orbc.load_example_large()
+    else:
+        v, e, f = orbc.evans_fung(34)
     return orbc.init_rbc(v, e, f, rho=1.05, seed=seed)
 
 
@@ -104,10 +113,12 @@ def force_error(sim32, sim64, ids):
 
 
 def timed_run(sim, steps: int, warmup: int):
+    """Returns wall seconds of `steps` steps, kernel timing, the observables
+    after the run, the rebuild count and the observables before it."""
     sim.run(warmup)
     sim.synchronize()
     sim.set_timing(True)
-    r0 = sim.observables()["rebuilds"]
+    obs0 = sim.observables()
     t0 = time.perf_counter()
     sim.run(steps)
     sim.synchronize()
@@ -115,7 +126,7 @@ def timed_run(sim, steps: int, warmup: int):
     tm = sim.timing()
     obs = sim.observables()
     sim.set_timing(False)
-    return t1 - t0, tm, obs, obs["rebuilds"] - r0
+    return t1 - t0, tm, obs, obs["rebuilds"] - obs0["rebuilds"], obs0
 
 
 def cpu_baseline(sysd, steps: int, n_lloyd: int, threads: int, timeout: int = 240):
@@ -185,7 +196,24 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-lloyd", type=int, default=8)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--mesh", choices=["example-large", "evans-fung"], default="example-large")
+    ap.add_argument("--volume-strength", type=float, default=0.05,
+                    help="constrain_volume strength (0: no volume constraint in the loop)")
+    ap.add_argument("--lloyd", type=int, default=64, help="Lloyd iterations of the Voronoi init")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run 3 plain device-to-device copies of MiB each (the reference kernel that "
+                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nb = args.copy_calibration << 20
+        a = torch.ones(nb // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "copies": 3}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md).", file=sys.stderr)
@@ -193,7 +221,7 @@ def main():
 
     import openrbc_amd as orbc
     t_build = time.perf_counter()
-    sysd = build_system(seed=7)
+    sysd = build_system(args.mesh, seed=7)
     n = len(sysd["lipid_x"]) + len(sysd["protein_x"])
     nb = len(sysd["bond_type"])
     print(f"# system: {len(sysd['lipid_x'])} lipids + {len(sysd['protein_x'])} proteins = {n}, {nb} bonds "
@@ -204,11 +232,17 @@ def main():
     upload(sim64, sysd)
     sim64.minimize(args.minimize)
     state = download(sim64)
-    sim32 = orbc.Simulation(precision=orbc.MIXED, seed=7, skin=args.skin)
+    vol = dict(volume_target=-1.0 if args.volume_strength != 0 else 0.0, volume_strength=args.volume_strength)
+    sim32 = orbc.Simulation(precision=orbc.MIXED, seed=7, skin=args.skin, **vol)
     upload(sim32, sysd, state)
     sim64.close()
-    sim64 = orbc.Simulation(precision=orbc.FP64, seed=7, skin=args.skin)
+    sim64 = orbc.Simulation(precision=orbc.FP64, seed=7, skin=args.skin, **vol)
     upload(sim64, sysd, state)
+    t_v = time.perf_counter()
+    for sim in (sim32, sim64):
+        sim.voronoi_init(0, args.lloyd)
+    print(f"# Voronoi cells: {len(sim32.voronoi_get()[0])} ({args.lloyd} Lloyd steps, "
+          f"{time.perf_counter() - t_v:.1f} s for both)", file=sys.stderr, flush=True)
 
     # force error of the mixed path against FP64 on the same configuration
     ids = np.concatenate([sysd['lipid_id'], sysd['protein_id']]).astype(np.int64)
@@ -220,16 +254,19 @@ def main():
     for name, sim in (("mixed", sim32), ("fp64", sim64)):
         if args.precision not in ("both", name):
             continue
-        wall, tm, obs, rebuilds = timed_run(sim, args.steps, args.warmup)
+        wall, tm, obs, rebuilds, obs0 = timed_run(sim, args.steps, args.warmup)
+        e0 = obs0["kinetic"] + obs0["potential"]
+        e1 = obs["kinetic"] + obs["potential"]
         kern_ms = tm["step_kernel_ms"] / max(1, tm["step_launches"])
         B = algorithmic_bytes(name, n, e_pairs, nb)
         res[name] = dict(value=n * args.steps / wall, ms_per_step=1e3 * wall / args.steps, kernel_ms=kern_ms,
                          kernel_launches=tm["step_launches"], rebuild_ms=tm["rebuild_ms"], rebuilds=rebuilds,
                          bytes_per_step=B, achieved_GBs=B / (kern_ms * 1e-3) / 1e9,
-                         temperature=obs["temperature"], potential_per_particle=obs["potential"] / n)
+                         temperature=obs["temperature"], potential_per_particle=obs["potential"] / n,
+                         energy_per_particle=dict(start=e0 / n, end=e1 / n, drift=(e1 - e0) / n))
         print(f"# {name}: {res[name]['value']:.4g} particle-steps/s, {res[name]['ms_per_step']:.3f} ms/step, "
               f"kernel {kern_ms:.3f} ms, rebuilds {rebuilds} ({tm['rebuild_ms']:.1f} ms), "
-              f"T={obs['temperature']:.4f}", file=sys.stderr, flush=True)
+              f"T={obs['temperature']:.4f}, E/N {e0 / n:.5f} -> {e1 / n:.5f}", file=sys.stderr, flush=True)
 
     # error again on the configuration reached by the mixed dynamics
     err_dyn = None
@@ -246,7 +283,7 @@ def main():
     head = res.get("mixed") or res.get("fp64")
     hname = "mixed" if "mixed" in res else "fp64"
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_step_{hname}_c2.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_step_langevin_{hname}_{args.mesh}.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -262,12 +299,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 forces / fp64 state" if hname == "mixed" else "fp64",
-        "data": "synthetic (OpenRBC example-large mesh populated by init_rbc rules, seed 7; Maxwell velocities)",
+        "data": f"synthetic ({'Evans-Fung biconcave mesh, geodesic nu=34' if args.mesh == 'evans-fung' else 'OpenRBC example-large mesh'} "
+                "populated by init_rbc rules, seed 7; FP64-minimised; Maxwell velocities)",
         "config": {
-            "workload": "full-size RBC, example-large mesh: 2,639,690 lipids + 565,816 proteins, 584,066 bonds; "
-                        "Langevin kT=0.22 dt=0.01 eta=0.01; skin rebuild",
+            "workload": f"full-size biconcave RBC ({args.mesh} mesh): {len(sysd['lipid_x'])} lipids + "
+                        f"{len(sysd['protein_x'])} proteins, {nb} bonds; Langevin kT=0.22 dt=0.01 eta=0.01; "
+                        f"skin rebuild; Voronoi update every 2 steps; volume constraint at the initial enclosed "
+                        f"volume (strength {args.volume_strength})",
             "n_particles": n, "n_bonds": nb, "pairs_within_cutoff": e_pairs, "skin": args.skin,
-            "precision": hname,
+            "precision": hname, "mesh": args.mesh,
         },
         "roofline": {
             "bound": "hbm",
@@ -284,7 +324,9 @@ def main():
         "force_error": {"minimised": err_min, "after_dynamics": err_dyn},
     }
     if "mixed" in res and "fp64" in res:
-        line["fp64"] = {k: res["fp64"][k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_GBs", "rebuilds")}
+        line["fp64"] = {k: res["fp64"][k] for k in ("value", "ms_per_step", "kernel_ms", "achieved_GBs", "rebuilds",
+                                                    "energy_per_particle")}
+        line["energy_per_particle"] = res["mixed"]["energy_per_particle"]
         line["fp64"]["roofline_frac"] = res["fp64"]["achieved_GBs"] * 1e9 / HBM_PEAK
         line["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
     print(json.dumps(line), flush=True)

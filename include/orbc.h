@@ -89,7 +89,9 @@ typedef struct orbc_config {
     int rebuild_period;         /* 0: rebuild on the skin criterion; k>0: every k steps */
     int neighbor_mode;          /* ORBC_NEIGHBOR_* */
     int thermostat;             /* ORBC_RUN_* */
-    double volume_target;       /* constrain_volume target; <=0 disables in orbc_run */
+    double volume_target;       /* volume constraint inside orbc_run: > 0 target volume, < 0 the
+                                   volume enclosed at the first orbc_run step, 0 off (needs
+                                   orbc_voronoi_init; cells updated every freq_voronoi steps) */
     double volume_strength;     /* constrain_volume strength */
     int device;                 /* HIP device ordinal */
     int block_size;             /* force/step kernel block size: 64/128/256/512; 0 = tuned default */

@@ -530,6 +530,7 @@ int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const
         if (species == ORBC_PROTEIN && n > 0 && !type) return fail(c, ORBC_ERR_INVALID, "protein type is required");
         Species &S = c->sp[species];
         const bool same = c->live && S.n == n;
+        c->vol_target_set = false;          // a volume target "< 0" is re-read from the new state
         S.n = n;
         S.x.assign(x, x + 3 * n);
         S.nrm.assign(nrm, nrm + 3 * n);
@@ -766,8 +767,37 @@ static void ev_pair(orbc_ctx *c, size_t k) {
     }
 }
 
+static bool volume_on(const orbc_ctx *c) { return c->cfg.volume_target != 0.0 && c->cfg.volume_strength != 0.0; }
+
+// Voronoi update every freq_voronoi steps (src/openrbc.cpp:200-205) and the
+// volume push of constrain_volume (src/constrain_volume.h:26-83) before the
+// force evaluation of step `step`.  Inside a batch the writes are gated on the
+// step's own skip condition; f_ext receives the push (or f when to_f).
+static void volume_prestep(orbc_ctx *c, long long step, bool gated, double *f_ext, bool to_f, bool cells = true) {
+    if (gated) {
+        c->voro.gstop = c->iflags;
+        c->voro.gslot = c->flags + 4 + (int)((step + 2) % 3);
+    }
+    if (cells && step % c->cfg.freq_voronoi == 0) voronoi_update_dev(c);
+    volume_push(c, c->vol_target, c->cfg.volume_strength, f_ext, to_f, false);
+    c->voro.gstop = nullptr;
+    c->voro.gslot = nullptr;
+}
+
+static void volume_begin(orbc_ctx *c) {
+    if (!volume_on(c) || c->vol_target_set) return;
+    if (c->cfg.volume_target > 0) {
+        c->vol_target = c->cfg.volume_target;
+    } else {
+        c->vol_target = volume_push(c, 1.0, 0.0, nullptr, false, true);
+        if (!(c->vol_target > 0)) throw std::runtime_error("enclosed volume is not positive");
+    }
+    c->vol_target_set = true;
+}
+
 static void run_langevin(orbc_ctx *c, long long nsteps) {
     long long remaining = nsteps;
+    const bool vol = volume_on(c);
     while (remaining > 0) {
         long long m = std::min<long long>(c->batch, remaining);
         if (c->cfg.rebuild_period > 0) m = std::min<long long>(m, std::max<long long>(1, c->cfg.rebuild_period - c->since_build));
@@ -775,8 +805,10 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
         const int cur0 = c->cur;
         if (c->timing) ev_pair(c, (size_t)m);
         for (long long s = 0; s < m; ++s) {
+            if (vol) volume_prestep(c, c->step + s, true, c->f_ext, false);
             StepArgs a = step_args(c);
             a.step = (unsigned long long)(c->step + s);
+            if (vol) a.f_ext = c->f_ext;
             if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s], c->stream));
             launch_step_langevin(a, c->mixed(), c->stream);
             if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s + 1], c->stream));
@@ -820,7 +852,14 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
 int orbc_run(orbc_ctx *c, long long nsteps) {
     return guarded(c, [&] {
         if (nsteps < 0) return fail(c, ORBC_ERR_INVALID, "nsteps < 0");
+        if (volume_on(c) && c->voro.ncell <= 0)
+            return fail(c, ORBC_ERR_STATE, "the volume constraint needs Voronoi cells: orbc_voronoi_init first");
         ensure_list(c);
+        try {
+            volume_begin(c);
+        } catch (const std::runtime_error &e) {
+            return fail(c, ORBC_ERR_INVALID, e.what());
+        }
         if (c->cfg.thermostat == ORBC_RUN_LANGEVIN) {
             try {
                 run_langevin(c, nsteps);
@@ -832,13 +871,14 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
             IntegArgs a;
             const double Q = nh_Q(c), nkT = 1.5 * (double)c->n * c->cfg.kBT;
             const long long nb = reduce_blocks(c->n);
-            auto forces = [&] {
+            auto forces = [&](bool cells) {
                 clear_ft(c, false);
                 ForceArgs fa = force_args(c);
                 launch_forces(fa, c->mixed(), false, true, c->nbonds > 0, c->stream);
+                if (volume_on(c)) volume_prestep(c, c->step, false, nullptr, true, cells);
             };
             if (!c->nh_started) {
-                forces();
+                forces(false);
                 a = integ_args(c);
                 launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
                 launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
@@ -852,7 +892,7 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 c->since_build++;
                 bool rb = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
                 if (rb) { rebuild(c); } else { ensure_list(c); }
-                forces();
+                forces(true);
                 a = integ_args(c);
                 launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
                 launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);

@@ -127,7 +127,13 @@ struct orbc_ctx {
         void *temp = nullptr;
         Grid cg;
         long long updates = 0;
+        bool aff_valid = false;            // aff holds an affiliation (warm start of the next partition)
+        const int *gstop = nullptr;        // skip gate of the batched run (VoroArgs::gstop)
+        const unsigned *gslot = nullptr;
     } voro;
+    // volume constraint inside orbc_run (cfg.volume_target != 0)
+    double vol_target = 0.0;
+    bool vol_target_set = false;
 
     bool mixed() const { return cfg.precision == ORBC_MIXED; }
 };
@@ -143,6 +149,8 @@ void ensure_list(orbc_ctx *c);
 void rebuild(orbc_ctx *c);
 void voronoi_free(orbc_ctx *c);
 void voronoi_rebuild_index(orbc_ctx *c);
+void voronoi_update_dev(orbc_ctx *c);
+double volume_push(orbc_ctx *c, double target, double strength, double *f_ext, bool to_f, bool read_back);
 }  // namespace orbc
 
 template <class F>

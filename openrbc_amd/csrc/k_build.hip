@@ -54,15 +54,21 @@ __global__ __launch_bounds__(BLOCK) void k_bbox_partial(const double *x, long lo
     }
 }
 
+// one wavefront: lanes stride over the block partials, then a shuffle reduction
 __global__ void k_bbox_final(const double *partial, int nb, double *out) {
-    if (threadIdx.x != 0) return;
     double r[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-    for (int b = 0; b < nb; ++b)
+    for (int b = threadIdx.x; b < nb; b += 64)
         for (int d = 0; d < 3; ++d) {
             r[d] = fmin(r[d], partial[6 * b + d]);
             r[3 + d] = fmax(r[3 + d], partial[6 * b + 3 + d]);
         }
-    for (int d = 0; d < 6; ++d) out[d] = r[d];
+    for (int off = 32; off > 0; off >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            r[d] = fmin(r[d], __shfl_xor(r[d], off, 64));
+            r[3 + d] = fmax(r[3 + d], __shfl_xor(r[3 + d], off, 64));
+        }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 6; ++d) out[d] = r[d];
 }
 
 __device__ inline int cell_coord(double v, double lo, double cs, int dim) {
@@ -202,33 +208,6 @@ __device__ inline bool nl_accept(const NlCtx &c, uint32_t j, float4 xj, int tj, 
     return true;
 }
 
-__device__ inline void nl_test(NlCtx &c, uint32_t j, float4 xj, int tj, const float *cut2tab) {
-    if (!nl_accept(c, j, xj, tj, cut2tab)) return;
-    if (c.cnt < c.a.kmax) c.a.nbr[(long long)c.cnt * c.a.stride + c.i] = j;
-    ++c.cnt;
-}
-
-// particle-centric scan of the cells at Chebyshev distance [rmin, rmax] from (cx,cy,cz)
-__device__ inline void nl_scan_cells(NlCtx &c, int cx, int cy, int cz, int rmin, int rmax, const float *cut2tab) {
-    const Grid &g = c.a.g;
-    for (int dz = -rmax; dz <= rmax; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= g.dim[2]) continue;
-        for (int dy = -rmax; dy <= rmax; ++dy) {
-            const int y = cy + dy;
-            if (y < 0 || y >= g.dim[1]) continue;
-            for (int dx = -rmax; dx <= rmax; ++dx) {
-                const int x = cx + dx;
-                if (x < 0 || x >= g.dim[0]) continue;
-                if (max(abs(dx), max(abs(dy), abs(dz))) < rmin) continue;
-                const long long cl = cell_lin(x, y, z, g);
-                const uint32_t e = c.a.cend[cl];
-                for (uint32_t j = c.a.cstart[cl]; j < e; ++j) nl_test(c, j, c.a.xq[j], c.a.type[j], cut2tab);
-            }
-        }
-    }
-}
-
 constexpr int NL_CAP = 256;          // candidates staged per wave per pass
 
 __device__ inline void wave_sync() {
@@ -240,6 +219,77 @@ __device__ inline void wave_sync() {
 __device__ inline int wave_max_i(int v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
     return v;
+}
+
+// Cells at Chebyshev distance 2..R around (cx,cy,cz), scanned by the whole
+// wave for ONE particle (held in c on every lane): 64 cells at a time have
+// their ranges prefix-summed across the lanes, their particles are staged in
+// LDS and tested 64 per round, hits appended after c.cnt in candidate order.
+// Only band-3 / actin (the sigma-3.4 LJ pair) reach past the first ring, so
+// this runs for a few percent of the particles.
+__device__ inline void nl_wave_shell(NlCtx &c, int cx, int cy, int cz, int R, float4 *sx, uint32_t *sj,
+                                     uint32_t *spref, uint32_t *scst, const float *cut2tab) {
+    const Grid &g = c.a.g;
+    const int lane = __lane_id();
+    const int D = 2 * R + 1, ncube = D * D * D;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int base = 0; base < ncube; base += 64) {
+        const int k = base + lane;
+        uint32_t st = 0, cntc = 0;
+        if (k < ncube) {
+            const int dx = k % D - R, dy = (k / D) % D - R, dz = k / (D * D) - R;
+            const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
+            if (max(abs(dx), max(abs(dy), abs(dz))) >= 2 && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.dim[0] &&
+                yy < g.dim[1] && zz < g.dim[2]) {
+                const long long cl = cell_lin(xx, yy, zz, g);
+                st = c.a.cstart[cl];
+                cntc = c.a.cend[cl] - st;
+            }
+        }
+        uint32_t inc = cntc;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        const uint32_t total = __shfl(inc, 63, 64);
+        wave_sync();
+        spref[lane + 1] = inc;
+        scst[lane] = st;
+        if (lane == 0) spref[0] = 0;
+        wave_sync();
+        for (uint32_t off = 0; off < total; off += NL_CAP) {
+            const uint32_t m = min((uint32_t)NL_CAP, total - off);
+            wave_sync();
+            for (uint32_t t = lane; t < m; t += 64) {
+                const uint32_t gidx = off + t;
+                int lo = 0, hi = 64;                // largest k with spref[k] <= gidx
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (spref[mid] <= gidx) lo = mid; else hi = mid;
+                }
+                const uint32_t j = scst[lo] + (gidx - spref[lo]);
+                sx[t] = c.a.xq[j];
+                sj[t] = j | ((uint32_t)c.a.type[j] << 29);
+            }
+            wave_sync();
+            for (uint32_t t0 = 0; t0 < m; t0 += 64) {
+                const uint32_t t = t0 + lane;
+                bool hit = false;
+                uint32_t j = 0;
+                if (t < m) {
+                    const uint32_t jt = sj[t];
+                    j = jt & 0x1fffffffu;
+                    hit = nl_accept(c, j, sx[t], (int)(jt >> 29), cut2tab);
+                }
+                const unsigned long long hm = __ballot(hit);
+                if (hit) {
+                    const int pos = c.cnt + __popcll(hm & lt);
+                    if (pos < c.a.kmax) c.a.nbr[(long long)pos * c.a.stride + c.i] = j;
+                }
+                c.cnt += __popcll(hm);
+            }
+        }
+    }
 }
 
 // first particle of every occupied cell (particles are sorted by cell)
@@ -272,6 +322,8 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
     __shared__ uint32_t s_j[BLOCK / 64][NL_CAP];
     __shared__ uint32_t s_pref[BLOCK / 64][28];
     __shared__ uint32_t s_cst[BLOCK / 64][27];
+    __shared__ uint32_t s_pref2[BLOCK / 64][65];
+    __shared__ uint32_t s_cst2[BLOCK / 64][64];
     __shared__ float s_cut2[36];
     if (threadIdx.x < 36) s_cut2[threadIdx.x] = c_list_cut2[threadIdx.x];
     __syncthreads();
@@ -361,12 +413,20 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
                 cnt += __popcll(hm & gmask);
             }
         }
-        if (act && stripe == 0) {
-            c.cnt = cnt;
-            if (c_rings[c.ti] >= 2) nl_scan_cells(c, cx, cy, cz, 2, c_rings[c.ti], s_cut2);
-            a.nnbr[i] = (uint32_t)(c.cnt < a.kmax ? c.cnt : a.kmax);
-            cnt = c.cnt;
+        // particles whose type reaches past the first ring: one at a time,
+        // the whole wave scans the outer shells for it
+        unsigned long long far = __ballot(act && stripe == 0 && c_rings[c.ti] >= 2);
+        while (far) {
+            const int src = __ffsll((long long)far) - 1;
+            far &= far - 1;
+            NlCtx cb{a, (long long)__shfl((int)i, src, 64), __shfl(c.ti, src, 64),
+                     make_float4(__shfl(c.xi.x, src, 64), __shfl(c.xi.y, src, 64), __shfl(c.xi.z, src, 64), 0.f),
+                     __shfl(c.ax, src, 64), __shfl(c.ay, src, 64), __shfl(c.az, src, 64),
+                     __shfl(c.c0, src, 64), __shfl(c.c1, src, 64), __shfl(c.c2, src, 64), __shfl(cnt, src, 64)};
+            nl_wave_shell(cb, cx, cy, cz, c_rings[cb.ti], s_x[w], s_j[w], s_pref2[w], s_cst2[w], s_cut2);
+            if (lane / S == src / S) cnt = cb.cnt;
         }
+        if (act && stripe == 0) a.nnbr[i] = (uint32_t)(cnt < a.kmax ? cnt : a.kmax);
         const int mc = wave_max_i((act && stripe == 0) ? cnt : 0);
         if (lane == 0 && mc > a.kmax) atomicMax(a.max_count, (unsigned)mc);
     }

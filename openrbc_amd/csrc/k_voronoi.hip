@@ -70,6 +70,8 @@ __global__ __launch_bounds__(BLOCK) void k_cgrid_ranges(const uint32_t *k, long 
     if (i + 1 == m || k[i] != k[i + 1]) end[k[i]] = (uint32_t)(i + 1);
 }
 
+__device__ inline bool voro_gated(const VoroArgs &a) { return a.gstop && (*a.gstop != 0 || *a.gslot != 0u); }
+
 // nearest centroid (lowest index on equal squared distance), grid rings outward
 __device__ int nearest_one(const VoroArgs &a, float p0, float p1, float p2) {
     const Grid &g = a.cg;
@@ -105,17 +107,46 @@ __device__ int nearest_one(const VoroArgs &a, float p0, float p1, float p2) {
     return bi;
 }
 
+// Same answer as nearest_one, seeded with a guess k0 (the previous
+// affiliation): every centroid at least as near as k0 lies in the ball of
+// radius |p - c_k0| around p, so only the grid cells overlapping that ball's
+// box are scanned (typically 1-4 instead of 27).  The radius carries a
+// relative margin so FP32 rounding of the box ends cannot drop a candidate.
+__device__ int nearest_warm(const VoroArgs &a, float p0, float p1, float p2, int k0) {
+    const Grid &g = a.cg;
+    float best = normsq_unfused(p0 - a.centroids[3 * k0], p1 - a.centroids[3 * k0 + 1], p2 - a.centroids[3 * k0 + 2]);
+    int bi = k0;
+    const float r = sqrtf(best) * 1.0001f + 1e-4f;
+    const int x0 = gcoord(p0 - r, g, 0), x1 = gcoord(p0 + r, g, 0);
+    const int y0 = gcoord(p1 - r, g, 1), y1 = gcoord(p1 + r, g, 1);
+    const int z0 = gcoord(p2 - r, g, 2), z1 = gcoord(p2 + r, g, 2);
+    for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y)
+            for (int x = x0; x <= x1; ++x) {
+                const long long cl = glin(x, y, z, g);
+                for (uint32_t q = a.cg_start[cl]; q < a.cg_end[cl]; ++q) {
+                    const int k = (int)a.cg_idx[q];
+                    const float d = normsq_unfused(p0 - a.centroids[3 * k], p1 - a.centroids[3 * k + 1],
+                                                   p2 - a.centroids[3 * k + 2]);
+                    if (d < best || (d == best && k < bi)) { best = d; bi = k; }
+                }
+            }
+    return bi;
+}
+
+// warm: out[] holds the previous affiliation and is used as the guess
 __global__ __launch_bounds__(BLOCK) void k_nearest(VoroArgs a, const float *q, long long nq, const double *qd,
-                                                   int32_t *out) {
+                                                   int32_t *out, int warm) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= nq) return;
+    if (i >= nq || voro_gated(a)) return;
     float p0, p1, p2;
     if (qd) {
         p0 = (float)qd[3 * i]; p1 = (float)qd[3 * i + 1]; p2 = (float)qd[3 * i + 2];
     } else {
         p0 = q[3 * i]; p1 = q[3 * i + 1]; p2 = q[3 * i + 2];
     }
-    out[i] = nearest_one(a, p0, p1, p2);
+    const int k0 = warm ? out[i] : -1;
+    out[i] = (k0 >= 0 && k0 < a.ncell) ? nearest_warm(a, p0, p1, p2, k0) : nearest_one(a, p0, p1, p2);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_within(VoroArgs a, const float *q, long long nq, float rmax, int cap,
@@ -146,21 +177,51 @@ __global__ __launch_bounds__(BLOCK) void k_within(VoroArgs a, const float *q, lo
     counts[i] = cnt;
 }
 
+// Segmented sum over runs of equal keys in consecutive lanes (particles are
+// Morton-ordered and cells compact, so a wave holds a few runs): the last lane
+// of each run adds the run's totals with one atomic per component.  Lanes with
+// key < 0 contribute nothing.  Integer sums are exact, so the grouping does not
+// change the result.
+__device__ inline void wave_run_add4(int key, unsigned long long v0, unsigned long long v1, unsigned long long v2,
+                                     unsigned long long v3, unsigned long long *sums) {
+    const int lane = __lane_id();
+    const int prev = __shfl_up(key, 1, 64), next = __shfl_down(key, 1, 64);
+    const bool head = lane == 0 || prev != key;
+    const bool tail = lane == 63 || next != key;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __clzll((long long)(heads & upto));      // first lane of my run
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long a0 = __shfl_up(v0, off, 64), a1 = __shfl_up(v1, off, 64);
+        const unsigned long long a2 = __shfl_up(v2, off, 64), a3 = __shfl_up(v3, off, 64);
+        if (lane - off >= start) { v0 += a0; v1 += a1; v2 += a2; v3 += a3; }
+    }
+    if (tail && key >= 0) {
+        atomicAdd(&sums[4 * key], v0);
+        atomicAdd(&sums[4 * key + 1], v1);
+        atomicAdd(&sums[4 * key + 2], v2);
+        atomicAdd(&sums[4 * key + 3], v3);
+    }
+}
+
 // fixed-point per-cell sums of lipid positions: sums[4k..4k+2] = sum x, [4k+3] = count
 __global__ __launch_bounds__(BLOCK) void k_centroid_sums(VoroArgs a, unsigned long long *sums) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n || a.type[i] != 0) return;
-    const int k = a.aff[i];
-    if (k < 0) return;
-    for (int d = 0; d < 3; ++d)
-        atomicAdd(&sums[4 * k + d], (unsigned long long)(long long)llrint(a.x[3 * i + d] * kPosScale));
-    atomicAdd(&sums[4 * k + 3], 1ull);
+    int k = -1;
+    unsigned long long v[3] = {0, 0, 0};
+    if (i < a.n && a.type[i] == 0) {
+        k = a.aff[i];
+        if (k >= 0)
+            for (int d = 0; d < 3; ++d) v[d] = (unsigned long long)(long long)llrint(a.x[3 * i + d] * kPosScale);
+    }
+    wave_run_add4(k, v[0], v[1], v[2], k >= 0 ? 1ull : 0ull, sums);
 }
 
 // update_centroid, src/voronoi.h:124-140 (empty cells keep their centroid)
-__global__ __launch_bounds__(BLOCK) void k_centroid_finish(const unsigned long long *sums, long long m, float *c) {
+__global__ __launch_bounds__(BLOCK) void k_centroid_finish(VoroArgs a, const unsigned long long *sums) {
     const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= m) return;
+    if (k >= a.ncell || voro_gated(a)) return;
+    float *c = a.centroids;
     const long long cnt = (long long)sums[4 * k + 3];
     if (cnt == 0) return;
     for (int d = 0; d < 3; ++d) {
@@ -172,12 +233,14 @@ __global__ __launch_bounds__(BLOCK) void k_centroid_finish(const unsigned long l
 // sum of lipid directors and lipid count per cell, src/constrain_volume.h:55
 __global__ __launch_bounds__(BLOCK) void k_normal_sums(VoroArgs a, unsigned long long *sums) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n || a.type[i] != 0) return;
-    const int k = a.aff[i];
-    if (k < 0) return;
-    for (int d = 0; d < 3; ++d)
-        atomicAdd(&sums[4 * k + d], (unsigned long long)(long long)llrint(a.nrm[3 * i + d] * kNrmScale));
-    atomicAdd(&sums[4 * k + 3], 1ull);
+    int k = -1;
+    unsigned long long v[3] = {0, 0, 0};
+    if (i < a.n && a.type[i] == 0) {
+        k = a.aff[i];
+        if (k >= 0)
+            for (int d = 0; d < 3; ++d) v[d] = (unsigned long long)(long long)llrint(a.nrm[3 * i + d] * kNrmScale);
+    }
+    wave_run_add4(k, v[0], v[1], v[2], k >= 0 ? 1ull : 0ull, sums);
 }
 
 __device__ inline void block_sum_store(double v, double *partial) {
@@ -231,7 +294,7 @@ __global__ __launch_bounds__(BLOCK) void k_centroid_center(const float *c, long 
 __global__ __launch_bounds__(BLOCK) void k_volume_push(VoroArgs a, const double *normals, const double *fpush,
                                                        double *f_ext, double *f64, float *f32) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n) return;
+    if (i >= a.n || voro_gated(a)) return;
     const int k = a.aff[i];
     double add[3] = {0, 0, 0};
     if (k >= 0) {
@@ -245,22 +308,29 @@ __global__ __launch_bounds__(BLOCK) void k_volume_push(VoroArgs a, const double 
     }
 }
 
-__global__ void k_volume_force(const double *vol_partial, long long nb, const double *center_partial, long long nbc,
-                               long long m, double target, double strength, double *scal) {
-    // scal[0] = V, scal[1] = push magnitude; scal[2..4] = center (used earlier)
-    if (threadIdx.x != 0) return;
-    double V = 0;
-    for (long long b = 0; b < nb; ++b) V += vol_partial[b];
-    scal[0] = V;
-    scal[1] = strength * (target - V) / target;
+__device__ inline double wave_sum_d(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
+// one wavefront: V = sum of the block partials, push = strength (target - V) / target
+__global__ void k_volume_force(const double *vol_partial, long long nb, double target, double strength, double *scal) {
+    double v = 0;
+    for (long long b = threadIdx.x; b < nb; b += 64) v += vol_partial[b];
+    v = wave_sum_d(v);
+    if (threadIdx.x == 0) {
+        scal[0] = v;
+        scal[1] = strength * (target - v) / target;
+    }
+}
+
+// one wavefront: centre of the centroids
 __global__ void k_center_finish(const double *partial, long long nb, long long m, double *center) {
-    if (threadIdx.x != 0) return;
     for (int d = 0; d < 3; ++d) {
-        double s = 0;
-        for (long long b = 0; b < nb; ++b) s += partial[d * nb + b];
-        center[d] = s / (double)m;
+        double v = 0;
+        for (long long b = threadIdx.x; b < nb; b += 64) v += partial[d * nb + b];
+        v = wave_sum_d(v);
+        if (threadIdx.x == 0) center[d] = v / (double)m;
     }
 }
 
@@ -278,8 +348,9 @@ void launch_cgrid_ranges(const uint32_t *sorted_keys, long long m, uint32_t *sta
     if (m > 0) hipLaunchKernelGGL(k_cgrid_ranges, dim3(nblk(m)), dim3(BLOCK), 0, s, sorted_keys, m, start, end);
 }
 
-void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out, hipStream_t s) {
-    if (nq > 0) hipLaunchKernelGGL(k_nearest, dim3(nblk(nq)), dim3(BLOCK), 0, s, a, q, nq, qd, out);
+void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out, bool warm,
+                    hipStream_t s) {
+    if (nq > 0) hipLaunchKernelGGL(k_nearest, dim3(nblk(nq)), dim3(BLOCK), 0, s, a, q, nq, qd, out, (int)warm);
 }
 
 void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, int cap, int *counts, int *out,
@@ -288,12 +359,12 @@ void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, 
 }
 
 void launch_centroid_sums(const VoroArgs &a, unsigned long long *sums, hipStream_t s) {
-    ORBC_HIP_CHECK(hipMemsetAsync(sums, 0, sizeof(unsigned long long) * 4 * (size_t)a.ncell, s));
+    launch_fill_u32(reinterpret_cast<uint32_t *>(sums), 0u, 8 * a.ncell, s);
     if (a.n > 0) hipLaunchKernelGGL(k_centroid_sums, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a, sums);
 }
 
-void launch_centroid_finish(const unsigned long long *sums, long long m, float *c, hipStream_t s) {
-    if (m > 0) hipLaunchKernelGGL(k_centroid_finish, dim3(nblk(m)), dim3(BLOCK), 0, s, sums, m, c);
+void launch_centroid_finish(const VoroArgs &a, const unsigned long long *sums, hipStream_t s) {
+    if (a.ncell > 0) hipLaunchKernelGGL(k_centroid_finish, dim3(nblk(a.ncell)), dim3(BLOCK), 0, s, a, sums);
 }
 
 void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals, double *partial, double *scal,
@@ -302,12 +373,11 @@ void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals,
     const long long nbc = nblk(a.ncell);
     hipLaunchKernelGGL(k_centroid_center, dim3((unsigned)nbc), dim3(BLOCK), 0, s, a.centroids, a.ncell, partial);
     hipLaunchKernelGGL(k_center_finish, dim3(1), dim3(64), 0, s, partial, nbc, a.ncell, scal + 2);
-    ORBC_HIP_CHECK(hipMemsetAsync(sums, 0, sizeof(unsigned long long) * 4 * (size_t)a.ncell, s));
+    launch_fill_u32(reinterpret_cast<uint32_t *>(sums), 0u, 8 * a.ncell, s);
     if (a.n > 0) hipLaunchKernelGGL(k_normal_sums, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a, sums);
     hipLaunchKernelGGL(k_volume_finish, dim3((unsigned)nbc), dim3(BLOCK), 0, s, sums, a.centroids, a.ncell, scal + 2,
                        normals, partial);
-    hipLaunchKernelGGL(k_volume_force, dim3(1), dim3(64), 0, s, partial, nbc, partial, nbc, a.ncell, target, strength,
-                       scal);
+    hipLaunchKernelGGL(k_volume_force, dim3(1), dim3(64), 0, s, partial, nbc, target, strength, scal);
 }
 
 void launch_volume_push(const VoroArgs &a, const double *normals, const double *fpush, double *f_ext, double *f64,

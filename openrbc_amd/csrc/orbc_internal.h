@@ -227,17 +227,22 @@ struct VoroArgs {
     // centroid grid (for nearest / within queries)
     Grid cg;
     const uint32_t *cg_start, *cg_end, *cg_idx;
+    // inside a batched run: the kernels that write persistent state (centroids,
+    // affiliation, push force) do nothing when the step they precede is skipped,
+    // i.e. when *gstop is set or the previous step raised its skin flag *gslot
+    const int *gstop;
+    const unsigned *gslot;
 };
 void launch_morton_ref(const float *pts, long long n, double bsize, uint32_t *keys, hipStream_t s);
 void launch_centroid_keys(const float *c, long long ncell, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s);
 void launch_cgrid_ranges(const uint32_t *sorted_keys, long long ncell, uint32_t *start, uint32_t *end,
                          hipStream_t s);
-void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out,
+void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out, bool warm,
                     hipStream_t s);
 void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, int cap, int *counts,
                    int *out, hipStream_t s);
 void launch_centroid_sums(const VoroArgs &a, unsigned long long *sums, hipStream_t s);
-void launch_centroid_finish(const unsigned long long *sums, long long ncell, float *centroids, hipStream_t s);
+void launch_centroid_finish(const VoroArgs &a, const unsigned long long *sums, hipStream_t s);
 void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals, double *partial, double *scal,
                    double target, double strength, hipStream_t s);
 void launch_volume_push(const VoroArgs &a, const double *normals, const double *fpush, double *f_ext,

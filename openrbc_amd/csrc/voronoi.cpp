@@ -68,6 +68,8 @@ VoroArgs voro_args(orbc_ctx *c) {
     a.cg_start = c->voro.cg_start;
     a.cg_end = c->voro.cg_end;
     a.cg_idx = c->voro.cg_idx;
+    a.gstop = c->voro.gstop;
+    a.gslot = c->voro.gslot;
     return a;
 }
 
@@ -92,17 +94,19 @@ void index_centroids(orbc_ctx *c) {
 
 void partition_all(orbc_ctx *c) {
     VoroArgs a = voro_args(c);
-    launch_nearest(a, nullptr, c->n, c->x[c->cur], c->voro.aff, c->stream);
+    launch_nearest(a, nullptr, c->n, c->x[c->cur], c->voro.aff, c->voro.aff_valid, c->stream);
+    c->voro.aff_valid = true;
 }
 
 void update_centroids(orbc_ctx *c) {
     VoroArgs a = voro_args(c);
     launch_centroid_sums(a, c->voro.sums, c->stream);
-    launch_centroid_finish(c->voro.sums, c->voro.ncell, c->voro.c, c->stream);
+    launch_centroid_finish(a, c->voro.sums, c->stream);
 }
 
 void voro_alloc(orbc_ctx *c, long long ncell) {
     voronoi_free(c);
+    c->vol_target_set = false;
     auto &V = c->voro;
     V.ncell = ncell;
     V.aff = c->mem.alloc<int32_t>(c->n);
@@ -322,7 +326,7 @@ int point_queries(orbc_ctx *c, long long npts, const float *pts, long long nq, c
     if (nearest) {
         int32_t *o = c->mem.alloc<int32_t>(nq);
         extra.push_back(o);
-        launch_nearest(a, dq, nq, nullptr, o, c->stream);
+        launch_nearest(a, dq, nq, nullptr, o, false, c->stream);
         ORBC_HIP_CHECK(hipMemcpyAsync(idx, o, 4 * nq, hipMemcpyDeviceToHost, c->stream));
     } else {
         int *dc = c->mem.alloc<int>(nq);

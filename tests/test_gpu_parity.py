@@ -248,3 +248,196 @@ def test_run_with_skin_rebuilds(precision, oracle):
     obs = sim.observables()
     assert obs["step"] == nsteps
     assert obs["rebuilds"] > 3
+
+
+# ---- Voronoi cells, stencil pair set, constrain_volume ---------------------------------
+def _voro_sim(s, mode=orbc.NEIGHBOR_EXACT, precision=orbc.FP64):
+    sim, lip, prot = make_sim(s, precision, neighbor_mode=mode)
+    rng = np.random.default_rng(1)
+    lipx = s["x"][lip]
+    c = lipx[rng.choice(len(lipx), max(1, len(lipx) // 14), replace=False)].astype(np.float32)
+    return sim, lip, prot, c
+
+
+def _aff(sim, lip, prot, n):
+    _, al, ap = sim.voronoi_get()
+    a = np.zeros(n, np.int32)
+    a[lip] = al
+    a[prot] = ap
+    return a
+
+
+def test_voronoi_partition_and_centroids(system, oracle):
+    s = system
+    sim, lip, prot, c = _voro_sim(s)
+    sim.voronoi_set_centroids(c)
+    aff = _aff(sim, lip, prot, len(s["type"]))
+    assert np.array_equal(aff, oracle.partition(s["x"], c))
+    sim.voronoi_update()              # centroids <- member means, then re-partition
+    c2, _, _ = sim.voronoi_get()
+    member = aff.copy()
+    member[s["type"] != 0] = -1
+    exp = c.copy()
+    for k in range(len(c)):
+        m = member == k
+        if m.any():
+            exp[k] = s["x"][m].mean(0)
+    assert np.abs(c2 - exp).max() < 2e-6 * max(1.0, np.abs(exp).max())
+    aff2 = _aff(sim, lip, prot, len(s["type"]))
+    assert np.array_equal(aff2, oracle.partition(s["x"], c2))
+
+
+def test_voronoi_lloyd_init_is_consistent(system, oracle):
+    s = system
+    sim, lip, prot, _ = _voro_sim(s)
+    sim.voronoi_init(0, 8)
+    c, _, _ = sim.voronoi_get()
+    assert len(c) == len(lip) // 14
+    aff = _aff(sim, lip, prot, len(s["type"]))
+    assert np.array_equal(aff, oracle.partition(s["x"], c))
+
+
+@pytest.mark.parametrize("precision", [orbc.FP64, orbc.MIXED])
+def test_voronoi_stencil_pair_set(system, precision, oracle):
+    """ORBC_NEIGHBOR_VORONOI: the reference's stencil pair set (centroid
+    distance < 6 / 8 / 9 for ll / pl / pp), src/compute_pairwise_fused.h:253-316."""
+    s = system
+    sim, lip, prot, c = _voro_sim(s, orbc.NEIGHBOR_VORONOI, precision)
+    sim.voronoi_set_centroids(c)
+    aff = _aff(sim, lip, prot, len(s["type"]))
+    got = sim.pairs()
+    if precision == orbc.FP64:
+        exp = id_pairs(s, oracle.pairs_voronoi(s["x"], s["type"], aff, c))
+        assert len(got) > 0 and np.array_equal(got, exp)
+        full = id_pairs(s, oracle.pairs_brute(s["x"], s["type"]))
+    else:
+        full = id_pairs(s, oracle.pairs_mixed(s["x"], s["type"], sim.observables()["anchor_h"]))
+    # the stencil keeps only in-cutoff pairs, and may drop some of them
+    fs = set(map(tuple, full.tolist()))
+    assert 0 < len(got) <= len(full) and all(tuple(p) in fs for p in got.tolist())
+
+
+def test_constrain_volume_vs_oracle(system, oracle):
+    s = system
+    sim, lip, prot, c = _voro_sim(s)
+    sim.voronoi_set_centroids(c)
+    aff = _aff(sim, lip, prot, len(s["type"]))
+    sim.compute_pairwise(zero_first=True)
+    f0, _ = gather(sim, lip, prot)
+    V = sim.constrain_volume(3.0, 0.05)
+    f1, _ = gather(sim, lip, prot)
+    fo = np.zeros_like(f0)
+    Vo = oracle.constrain_volume(s["n"], s["type"], aff, c, 3.0, 0.05, fo)
+    assert abs(V - Vo) < 1e-10 * max(1.0, abs(Vo))
+    assert np.abs((f1 - f0) - fo).max() < 1e-10 * max(1.0, np.abs(fo).max())
+
+
+def test_volume_of_a_lipid_sphere():
+    """Cone-sum volume of a lipid sphere (radial directors) against the closed form."""
+    sphere = orbc.random_sphere(r=40.0, seed=3)
+    sim = orbc.Simulation(precision=orbc.FP64)
+    orbc.system_to_simulation(sphere, sim)
+    sim.voronoi_init(0, 6)
+    V = sim.constrain_volume(1.0, 0.0)
+    # every cell contributes h * count * (pi 1.26 / 4) / 3 * 1e-6 with h ~ r (cap sagitta < 0.1 %)
+    n = len(sphere["lipid_x"])
+    exp = 40.0 * n * 3.1415926 * 1.26 / 12.0 * 1e-6
+    assert abs(V / exp - 1) < 0.01
+
+
+# ---- C1: icosphere level 3 through the initialiser (225,384 particles) -----------------
+@pytest.fixture(scope="module")
+def c1():
+    v, e, f = orbc.icosphere(3)
+    s = orbc.init_rbc(v, e, f, seed=1)
+    rng = np.random.default_rng(1)
+    from openrbc_amd.systems import tilt_directors
+    ln = tilt_directors(s["lipid_n"], 0.01, rng)
+    x = np.vstack([s["lipid_x"], s["protein_x"]])
+    n = np.vstack([ln, s["protein_n"]])
+    t = np.concatenate([np.zeros(len(s["lipid_x"]), np.int32), s["protein_type"]])
+    ids = np.concatenate([s["lipid_id"], s["protein_id"]])
+    idx = {int(k): i for i, k in enumerate(ids)}
+    bi = np.array([idx[int(k)] for k in s["bond_i"]]); bj = np.array([idx[int(k)] for k in s["bond_j"]])
+    return dict(x=x, n=n, type=t, id=ids, v=np.zeros_like(x), o=np.zeros_like(x),
+                bonds=(s["bond_type"], bi, bj))
+
+
+@pytest.mark.parametrize("precision,tol", [(orbc.FP64, EPS_FP64), (orbc.MIXED, 5e-5)])
+def test_c1_forces_vs_oracle(c1, precision, tol, oracle):
+    s = c1
+    sim, lip, prot = make_sim(s, precision)
+    sim.compute_pairwise(zero_first=True)
+    sim.compute_bonded()
+    f, t = gather(sim, lip, prot)
+    pairs = oracle.pairs_cell(s["x"], s["type"])
+    got = sim.pairs()
+    if precision == orbc.FP64:
+        assert np.array_equal(got, id_pairs(s, pairs))
+    else:
+        # FP32 selection may flip pairs that sit on a cutoff to within FP32 rounding
+        e = set(map(tuple, id_pairs(s, pairs).tolist()))
+        g = set(map(tuple, got.tolist()))
+        flips = np.array(sorted(e ^ g), np.int64).reshape(-1, 2)
+        assert len(flips) <= 1e-5 * len(e)
+        order = np.argsort(s["id"])
+        to_idx = lambda q: order[np.searchsorted(s["id"][order], q)]
+        if len(flips):
+            a, b = to_idx(flips[:, 0]), to_idx(flips[:, 1])
+            r = np.linalg.norm(s["x"][a] - s["x"][b], axis=1)
+            ff = oracle.ff()
+            ta, tb = s["type"][a], s["type"][b]
+            cuts = np.stack([np.sqrt(ff["cut2_poly"][ta, tb]), np.sqrt(ff["cut2_lj"][ta, tb])], 1)
+            assert np.all(np.min(np.abs(r[:, None] / cuts - 1), 1) < 1e-5)
+        pairs = np.stack([to_idx(got[:, 0]), to_idx(got[:, 1])], 1)
+    of, ot, _, _ = oracle.forces(s["x"], s["n"], s["type"], pairs=pairs, bonds=s["bonds"])
+    assert eps_f(f, of) < tol
+    assert eps_f(t, ot) < tol
+
+
+@pytest.mark.parametrize("skin", [0.4, 0.03])
+def test_run_with_volume_constraint_matches_call_sequence(skin):
+    """orbc_run with the volume constraint == the per-step calls of the
+    reference loop (src/openrbc.cpp:200-233 with constrain_volume enabled):
+    Voronoi update every freq_voronoi steps, forces, constrain_volume, then
+    integrate(verlet_langevin).  skin=0.03 forces rebuilds inside the batches,
+    so the skip gate of the cell/volume kernels is exercised too."""
+    sphere = orbc.random_sphere(r=12.0, seed=2)
+
+    def make(target):
+        sim = orbc.Simulation(precision=orbc.FP64, seed=11, skin=skin, kBT=0.22, volume_target=target,
+                              volume_strength=0.5)
+        orbc.system_to_simulation(sphere, sim, kT=0.22, tilt=0.02, seed=3)
+        sim.voronoi_init(0, 4)
+        return sim
+
+    call_sim = make(0.0)
+    # target 2% above the initial enclosed volume, so that the push is not zero
+    V0 = 1.02 * call_sim.constrain_volume(1.0, 0.0)
+    run_sim = make(V0)
+    nsteps = 40
+    run_sim.run(nsteps)
+    for step in range(nsteps):
+        if step % 2 == 0:
+            call_sim.voronoi_update()
+        call_sim.compute_pairwise(zero_first=True)
+        call_sim.compute_bonded()
+        call_sim.constrain_volume(V0, 0.5)
+        call_sim.integrate("verlet_langevin")
+    a = run_sim.get_state(orbc.LIPID)
+    b = call_sim.get_state(orbc.LIPID)
+    for k in ("x", "v", "n"):
+        assert np.abs(a[k] - b[k]).max() < 1e-8 * max(1.0, np.abs(b[k]).max()), k
+    ca, _, _ = run_sim.voronoi_get()
+    cb, _, _ = call_sim.voronoi_get()
+    assert np.abs(ca - cb).max() < 1e-4
+    if skin < 0.1:
+        assert run_sim.observables()["rebuilds"] > 3
+
+
+def test_run_volume_needs_cells():
+    sphere = orbc.random_sphere(r=8.0, seed=2)
+    sim = orbc.Simulation(precision=orbc.FP64, volume_target=-1.0)
+    orbc.system_to_simulation(sphere, sim)
+    with pytest.raises(orbc.OrbcError):
+        sim.run(2)
