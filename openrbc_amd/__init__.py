@@ -20,7 +20,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "liborbc_hip.so")
+# ORBC_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("ORBC_LIBRARY") or os.path.join(HERE, "_lib", "liborbc_hip.so")
 
 # enums of include/orbc.h
 LIPID, PROTEIN = 0, 1

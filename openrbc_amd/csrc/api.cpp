@@ -69,7 +69,7 @@ ForceArgs force_args(orbc_ctx *c) {
     ForceArgs a{};
     a.n = c->n; a.stride = c->stride;
     a.x = c->x[c->cur]; a.nrm = c->nrm[c->cur];
-    a.xq = c->xq[c->cur]; a.nq = c->nq[c->cur];
+    a.pq = c->pq[c->cur];
     a.type = c->type; a.id = c->id;
     a.nbr = c->nbr; a.nnbr = c->nnbr;
     a.bnd = c->nbonds ? c->bnd : nullptr;
@@ -87,8 +87,7 @@ IntegArgs integ_args(orbc_ctx *c) {
     a.x = c->x[c->cur]; a.v = c->v; a.nrm = c->nrm[c->cur]; a.o = c->o;
     a.f32 = c->f32; a.t32 = c->t32; a.f64 = c->f64; a.t64 = c->t64;
     a.type = c->type; a.id = c->id; a.anc = c->anc;
-    a.xq = c->mixed() ? c->xq[c->cur] : nullptr;
-    a.nq = c->mixed() ? c->nq[c->cur] : nullptr;
+    a.pq = c->mixed() ? c->pq[c->cur] : nullptr;
     a.anc_grid = c->ag;
     a.dt = c->cfg.dt;
     langevin_coeffs(c->cfg, a.gamma, a.sigma);
@@ -110,8 +109,7 @@ void alloc_store(orbc_ctx *c) {
     for (int b = 0; b < 2; ++b) {
         c->x[b] = m.alloc<double>(3 * n);
         c->nrm[b] = m.alloc<double>(3 * n);
-        c->xq[b] = m.alloc<float4>(n);
-        c->nq[b] = m.alloc<float4>(n);
+        c->pq[b] = m.alloc<PQ>(n);
     }
     c->v = m.alloc<double>(3 * n); c->o = m.alloc<double>(3 * n);
     c->v_alt = m.alloc<double>(3 * n); c->o_alt = m.alloc<double>(3 * n);
@@ -374,7 +372,7 @@ void rebuild(orbc_ctx *c) {
     if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
     launch_inverse(c->slot, c->inv, n, s);
     // 6. mirrors, anchors, build-time offsets
-    launch_refresh(n, c->x[c->cur], c->nrm[c->cur], c->type, ag, c->xq[c->cur], c->nq[c->cur], c->rb, c->anc, s);
+    launch_refresh(n, c->x[c->cur], c->nrm[c->cur], c->type, ag, c->pq[c->cur], c->rb, c->anc, s);
     // 7. cell ranges and the Verlet list
     launch_cell_ranges(c->x[c->cur], n, g, c->cstart, c->cend, s);
     // occupied cells (the sort buffers are free again after the permutation)
@@ -386,7 +384,7 @@ void rebuild(orbc_ctx *c) {
         }
         NlistArgs a{};
         a.n = n; a.stride = c->stride; a.x = c->x[c->cur]; a.type = c->type; a.g = g;
-        a.xq = c->xq[c->cur]; a.anc = ag;
+        a.pq = c->pq[c->cur]; a.anc = ag;
         for (int d = 0; d < 3; ++d) { a.box.lo[d] = c->cfg.box_lo[d]; a.box.hi[d] = c->cfg.box_hi[d];
                                       a.box.L[d] = c->cfg.box_hi[d] - c->cfg.box_lo[d]; a.box.per[d] = 0; }
         a.cstart = c->cstart; a.cend = c->cend; a.nbr = c->nbr; a.nnbr = c->nnbr; a.kmax = c->kmax;
@@ -408,6 +406,8 @@ void rebuild(orbc_ctx *c) {
     zero_flag(c, 0);
     ORBC_HIP_CHECK(hipMemsetAsync(c->flags + 4, 0, 3 * sizeof(unsigned), s));
     ORBC_HIP_CHECK(hipMemsetAsync(c->iflags, 0, sizeof(int), s));
+    // 9. centroid grid of the Voronoi cells (kept between rebuilds, see VoroArgs::cdisp)
+    voronoi_rebuild_index(c);
     c->need_rebuild = false;
     c->moved = false;
     c->since_build = 0;
@@ -554,8 +554,8 @@ int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const
                 ORBC_HIP_CHECK(hipMemcpyAsync(c->stage + 3 * off, src[f]->data(), 24 * n, hipMemcpyHostToDevice, c->stream));
                 launch_gather_slots(c->n, c->slot, c->stage, fields[f], c->stream);
             }
-            launch_refresh(c->n, c->x[c->cur], c->nrm[c->cur], c->type, c->ag, c->mixed() ? c->xq[c->cur] : nullptr,
-                           c->mixed() ? c->nq[c->cur] : nullptr, c->stagef, c->anc, c->stream);
+            launch_refresh(c->n, c->x[c->cur], c->nrm[c->cur], c->type, c->ag, c->mixed() ? c->pq[c->cur] : nullptr,
+                           c->stagef, c->anc, c->stream);
             c->need_rebuild = true;
             sync(c);
         } else if (c->live) {
@@ -736,8 +736,8 @@ static StepArgs step_args(orbc_ctx *c) {
     a.x_in = c->x[in]; a.n_in = c->nrm[in];
     a.x_out = c->x[out]; a.n_out = c->nrm[out];
     a.v = c->v; a.o = c->o;
-    a.xq_in = c->xq[in]; a.nq_in = c->nq[in];
-    a.xq_out = c->xq[out]; a.nq_out = c->nq[out];
+    a.pq_in = c->pq[in];
+    a.pq_out = c->pq[out];
     a.type = c->type; a.id = c->id; a.anc = c->anc; a.rb = c->rb;
     a.nbr = c->nbr; a.nnbr = c->nnbr;
     a.bnd = c->nbonds ? c->bnd : nullptr;
@@ -770,18 +770,21 @@ static void ev_pair(orbc_ctx *c, size_t k) {
 static bool volume_on(const orbc_ctx *c) { return c->cfg.volume_target != 0.0 && c->cfg.volume_strength != 0.0; }
 
 // Voronoi update every freq_voronoi steps (src/openrbc.cpp:200-205) and the
-// volume push of constrain_volume (src/constrain_volume.h:26-83) before the
-// force evaluation of step `step`.  Inside a batch the writes are gated on the
-// step's own skip condition; f_ext receives the push (or f when to_f).
-static void volume_prestep(orbc_ctx *c, long long step, bool gated, double *f_ext, bool to_f, bool cells = true) {
+// volume evaluation of constrain_volume (src/constrain_volume.h:26-83) before
+// the force evaluation of step `step`.  Inside a batch every kernel of it is
+// gated on the step's own skip condition, and the fused step kernel applies
+// the push; otherwise (to_f) the push is added to the force accumulators.
+static void volume_prestep(orbc_ctx *c, long long step, bool gated, bool to_f, bool cells = true) {
     if (gated) {
-        c->voro.gstop = c->iflags;
-        c->voro.gslot = c->flags + 4 + (int)((step + 2) % 3);
+        c->voro.gate.stop = c->iflags;
+        c->voro.gate.slot = c->flags + 4 + (int)((step + 2) % 3);
     }
     if (cells && step % c->cfg.freq_voronoi == 0) voronoi_update_dev(c);
-    volume_push(c, c->vol_target, c->cfg.volume_strength, f_ext, to_f, false);
-    c->voro.gstop = nullptr;
-    c->voro.gslot = nullptr;
+    if (to_f)
+        volume_push(c, c->vol_target, c->cfg.volume_strength, nullptr, true, false);
+    else
+        volume_eval(c, c->vol_target, c->cfg.volume_strength);
+    c->voro.gate = Gate{};
 }
 
 static void volume_begin(orbc_ctx *c) {
@@ -800,15 +803,28 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
     const bool vol = volume_on(c);
     while (remaining > 0) {
         long long m = std::min<long long>(c->batch, remaining);
-        if (c->cfg.rebuild_period > 0) m = std::min<long long>(m, std::max<long long>(1, c->cfg.rebuild_period - c->since_build));
+        if (c->cfg.rebuild_period > 0) {
+            m = std::min<long long>(m, std::max<long long>(1, c->cfg.rebuild_period - c->since_build));
+        } else if (c->list_life > 0) {
+            // launch about as many steps as the list is expected to last: steps
+            // after a skin trigger are skipped, but each still costs its launches
+            // (and the Voronoi/volume work queued before it)
+            const long long margin = vol ? 1 : 4;
+            const long long left = (long long)c->list_life - c->since_build + margin;
+            m = std::min<long long>(m, std::max<long long>(1, left));
+        }
         ORBC_HIP_CHECK(hipMemsetAsync(c->iflags + 1, 0, sizeof(int), c->stream));
         const int cur0 = c->cur;
         if (c->timing) ev_pair(c, (size_t)m);
         for (long long s = 0; s < m; ++s) {
-            if (vol) volume_prestep(c, c->step + s, true, c->f_ext, false);
+            if (vol) volume_prestep(c, c->step + s, true, false);
             StepArgs a = step_args(c);
             a.step = (unsigned long long)(c->step + s);
-            if (vol) a.f_ext = c->f_ext;
+            if (vol) {
+                a.push = c->scal + 9;
+                a.cell_normal = c->voro.normals;
+                a.aff = c->voro.aff;
+            }
             if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s], c->stream));
             launch_step_langevin(a, c->mixed(), c->stream);
             if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s + 1], c->stream));
@@ -833,6 +849,11 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
         remaining -= done;
         if (bad) throw std::runtime_error("non-finite state in orbc_run");
         const bool period = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
+        if (fl[0]) {
+            // running estimate of how many steps a list lasts (skin criterion)
+            c->list_life = c->list_life > 0 ? 0.75 * c->list_life + 0.25 * (double)c->since_build
+                                            : (double)c->since_build;
+        }
         if (fl[0] || period) {
             if (c->timing) { ev_pair(c, 0); ORBC_HIP_CHECK(hipEventRecord(c->ev[0], c->stream)); }
             rebuild(c);
@@ -875,7 +896,7 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 clear_ft(c, false);
                 ForceArgs fa = force_args(c);
                 launch_forces(fa, c->mixed(), false, true, c->nbonds > 0, c->stream);
-                if (volume_on(c)) volume_prestep(c, c->step, false, nullptr, true, cells);
+                if (volume_on(c)) volume_prestep(c, c->step, false, true, cells);
             };
             if (!c->nh_started) {
                 forces(false);

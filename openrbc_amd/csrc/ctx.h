@@ -16,6 +16,8 @@ using orbc::Box;
 using orbc::FFTab;
 using orbc::Grid;
 using orbc::HipError;
+using orbc::PQ;
+using orbc::Gate;
 
 struct DevArena {
     std::vector<void *> ptrs;
@@ -71,7 +73,7 @@ struct orbc_ctx {
     double *fe = nullptr, *te = nullptr;     // scratch force buffers for energy evaluation
     float *fe32 = nullptr, *te32 = nullptr;
     double *f_ext = nullptr;                 // volume-constraint force for orbc_run
-    float4 *xq[2] = {nullptr, nullptr}, *nq[2] = {nullptr, nullptr};
+    PQ *pq[2] = {nullptr, nullptr};
     float *rb = nullptr;
     uint32_t *anc = nullptr;
     uint8_t *type = nullptr, *type_alt = nullptr;
@@ -128,9 +130,11 @@ struct orbc_ctx {
         Grid cg;
         long long updates = 0;
         bool aff_valid = false;            // aff holds an affiliation (warm start of the next partition)
-        const int *gstop = nullptr;        // skip gate of the batched run (VoroArgs::gstop)
-        const unsigned *gslot = nullptr;
+        float *cref = nullptr;             // centroids as binned in the centroid grid
+        unsigned *cdisp = nullptr;         // max squared drift since (float bits)
+        Gate gate;                         // skip gate of the batched run (VoroArgs::gate)
     } voro;
+    double list_life = 0.0;       // steps a neighbour list lasts on average (batch sizing)
     // volume constraint inside orbc_run (cfg.volume_target != 0)
     double vol_target = 0.0;
     bool vol_target_set = false;
@@ -151,6 +155,7 @@ void voronoi_free(orbc_ctx *c);
 void voronoi_rebuild_index(orbc_ctx *c);
 void voronoi_update_dev(orbc_ctx *c);
 double volume_push(orbc_ctx *c, double target, double strength, double *f_ext, bool to_f, bool read_back);
+void volume_eval(orbc_ctx *c, double target, double strength);
 }  // namespace orbc
 
 template <class F>

@@ -129,8 +129,8 @@ __global__ __launch_bounds__(BLOCK) void k_inverse(const uint32_t *slot, uint32_
 }
 
 __global__ __launch_bounds__(BLOCK) void k_refresh(long long n, const double *x, const double *nrm,
-                                                   const uint8_t *type, Anchor a, float4 *xq, float4 *nq,
-                                                   float *rb, uint32_t *anc) {
+                                                   const uint8_t *type, Anchor a, PQ *pq, float *rb,
+                                                   uint32_t *anc) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     long long A[3];
@@ -147,10 +147,12 @@ __global__ __launch_bounds__(BLOCK) void k_refresh(long long n, const double *x,
     rb[3 * i] = r[0];
     rb[3 * i + 1] = r[1];
     rb[3 * i + 2] = r[2];
-    if (xq) {
-        xq[i] = make_float4(r[0], r[1], r[2], __uint_as_float(w));
-        nq[i] = make_float4(__double2float_rn(nrm[3 * i]), __double2float_rn(nrm[3 * i + 1]),
-                            __double2float_rn(nrm[3 * i + 2]), __int_as_float((int)type[i]));
+    if (pq) {
+        PQ q;
+        q.x = make_float4(r[0], r[1], r[2], __uint_as_float(w));
+        q.n = make_float4(__double2float_rn(nrm[3 * i]), __double2float_rn(nrm[3 * i + 1]),
+                          __double2float_rn(nrm[3 * i + 2]), __int_as_float((int)type[i]));
+        pq[i] = q;
     }
 }
 
@@ -268,7 +270,7 @@ __device__ inline void nl_wave_shell(NlCtx &c, int cx, int cy, int cz, int R, fl
                     if (spref[mid] <= gidx) lo = mid; else hi = mid;
                 }
                 const uint32_t j = scst[lo] + (gidx - spref[lo]);
-                sx[t] = c.a.xq[j];
+                sx[t] = c.a.pq[j].x;
                 sj[t] = j | ((uint32_t)c.a.type[j] << 29);
             }
             wave_sync();
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
         NlCtx c{a, i, 0, make_float4(0, 0, 0, 0), 0, 0, 0, 0.f, 0.f, 0.f, 0};
         if (act) {
             c.ti = a.type[i];
-            c.xi = a.xq[i];
+            c.xi = a.pq[i].x;
             const uint32_t wi = __float_as_uint(c.xi.w);
             c.ax = (int)(wi & a.anc.mx); c.ay = (int)((wi >> a.anc.shy) & a.anc.my); c.az = (int)(wi >> a.anc.shz);
             if (a.aff) {
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
                     if (s_pref[w][mid] <= gidx) lo = mid; else hi = mid;
                 }
                 const uint32_t j = s_cst[w][lo] + (gidx - s_pref[w][lo]);
-                s_x[w][t] = a.xq[j];
+                s_x[w][t] = a.pq[j].x;
                 s_j[w][t] = j | ((uint32_t)a.type[j] << 29);
             }
             wave_sync();
@@ -506,10 +508,10 @@ void launch_inverse(const uint32_t *slot, uint32_t *inv, long long n, hipStream_
     hipLaunchKernelGGL(k_inverse, dim3(nblk(n)), dim3(BLOCK), 0, s, slot, inv, n);
 }
 
-void launch_refresh(long long n, const double *x, const double *nrm, const uint8_t *type, Anchor a, float4 *xq,
-                    float4 *nq, float *rb, uint32_t *anc, hipStream_t s) {
+void launch_refresh(long long n, const double *x, const double *nrm, const uint8_t *type, Anchor a, PQ *pq,
+                    float *rb, uint32_t *anc, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_refresh, dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, nrm, type, a, xq, nq, rb, anc);
+    hipLaunchKernelGGL(k_refresh, dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, nrm, type, a, pq, rb, anc);
 }
 
 void launch_cell_ranges(const double *x, long long n, Grid g, uint32_t *cstart, uint32_t *cend, hipStream_t s) {

@@ -183,13 +183,27 @@ __device__ inline void add_to(Acc<double> &o, const Acc<R> &a) {
     o.e += (double)a.e; o.w += (double)a.w;
 }
 
-constexpr int UNROLL = 4;   // neighbours gathered per group (memory-level parallelism)
+// Tuning knobs (compile-time): neighbours gathered per group (memory-level
+// parallelism per wave) and, if nonzero, the waves per SIMD the fused step
+// kernel's register budget is capped for.
+#ifndef ORBC_STEP_UNROLL
+#define ORBC_STEP_UNROLL 4
+#endif
+#ifndef ORBC_STEP_WAVES
+#define ORBC_STEP_WAVES 0
+#endif
+constexpr int UNROLL = ORBC_STEP_UNROLL;
+#if ORBC_STEP_WAVES > 0
+#define ORBC_STEP_ATTR __attribute__((amdgpu_waves_per_eu(ORBC_STEP_WAVES)))
+#else
+#define ORBC_STEP_ATTR
+#endif
 
 // ---- per-particle force accumulation ---------------------------------------
 // Mixed: FP32 pair terms from the FP32 mirrors, summed in FP32 over each group
 // of UNROLL neighbours and in FP64 across groups.
 template <bool ENERGY>
-__device__ __forceinline__ void accumulate_mixed(long long i, long long stride, const float4 *xq, const float4 *nq,
+__device__ __forceinline__ void accumulate_mixed(long long i, long long stride, const PQ *pq,
                                                  const uint32_t *nbr, const uint32_t *nnbr, const uint32_t *bnd,
                                                  const uint8_t *nbnd, const Anchor &ag, float4 xi, float4 ni,
                                                  const FFS<float> &t, Acc<double> &out, bool pair, bool bond) {
@@ -204,7 +218,7 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
             for (int u = 0; u < UNROLL; ++u) jj[u] = (k0 + u < cnt) ? nb[(long long)(k0 + u) * stride] : (uint32_t)i;
             float4 xj[UNROLL], nj[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) { xj[u] = xq[jj[u]]; nj[u] = nq[jj[u]]; }
+            for (int u = 0; u < UNROLL; ++u) { xj[u] = pq[jj[u]].x; nj[u] = pq[jj[u]].n; }
             Acc<float> acc;
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
@@ -225,7 +239,7 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
         for (int k = 0; k < nb; ++k) {
             const uint32_t e = bnd[(long long)k * stride + i];
             const uint32_t j = e & BOND_INDEX_MASK;
-            const float4 xj = xq[j];
+            const float4 xj = pq[j].x;
             const AnchorI aj = unpack(__float_as_uint(xj.w), ag);
             const float dx = (xj.x - xi.x) + ag.h * (float)(aj.x - ai.x);
             const float dy = (xj.y - xi.y) + ag.h * (float)(aj.y - ai.y);
@@ -287,7 +301,7 @@ __global__ __launch_bounds__(BLOCK) void k_forces(ForceArgs a, int pair, int bon
     if (i >= a.n) return;
     Acc<double> acc;
     if constexpr (MIXED) {
-        accumulate_mixed<ENERGY>(i, a.stride, a.xq, a.nq, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, a.xq[i], a.nq[i],
+        accumulate_mixed<ENERGY>(i, a.stride, a.pq, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, a.pq[i].x, a.pq[i].n,
                                  t, acc, pair, bond);
         a.f32[3 * i] += (float)acc.fx; a.f32[3 * i + 1] += (float)acc.fy; a.f32[3 * i + 2] += (float)acc.fz;
         a.t32[3 * i] += (float)acc.tx; a.t32[3 * i + 1] += (float)acc.ty; a.t32[3 * i + 2] += (float)acc.tz;
@@ -313,13 +327,13 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_export(ForceArgs a, unsigned *i
         if (!(idi < idj)) continue;
         bool hit;
         if (MIXED) {
-            const float4 xi = a.xq[i], xj = a.xq[j];
+            const float4 xi = a.pq[i].x, xj = a.pq[j].x;
             const AnchorI ai = unpack(__float_as_uint(xi.w), a.anc_grid), aj = unpack(__float_as_uint(xj.w), a.anc_grid);
             const float dx = add_rn(sub_rn(xi.x, xj.x), a.anc_grid.h * (float)(ai.x - aj.x));
             const float dy = add_rn(sub_rn(xi.y, xj.y), a.anc_grid.h * (float)(ai.y - aj.y));
             const float dz = add_rn(sub_rn(xi.z, xj.z), a.anc_grid.h * (float)(ai.z - aj.z));
             const float r2 = r2_rn(dx, dy, dz);
-            const int kk = __float_as_int(a.nq[i].w) * NT + __float_as_int(a.nq[j].w);
+            const int kk = __float_as_int(a.pq[i].n.w) * NT + __float_as_int(a.pq[j].n.w);
             hit = r2 > 1e-5f && (r2 < c_fff.cut2_poly[kk] || r2 < c_fff.cut2_lj[kk]);
         } else {
             const double dx = sub_rn(a.x[3 * i], a.x[3 * (long long)j]);
@@ -355,7 +369,7 @@ __device__ inline void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, u
 // integrate(verlet_langevin)) in one pass; the Langevin update restates
 // src/integrate_langevin.h:275-297 in FP64 with counter-based noise.
 template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
+__global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs a) {
     const int k3 = (int)(a.step % 3ull);
     // skin flag of this step's input positions; no block of this launch writes it
     if (*a.stop || a.dslot[(k3 + 2) % 3] != 0u) {
@@ -378,9 +392,9 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
         uint32_t wi;
         Acc<double> acc;
         if constexpr (MIXED) {
-            const float4 xi4 = a.xq_in[i];
-            const float4 ni4 = a.nq_in[i];
-            accumulate_mixed<false>(i, a.stride, a.xq_in, a.nq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, xi4,
+            const PQ qi = a.pq_in[i];
+            const float4 xi4 = qi.x, ni4 = qi.n;
+            accumulate_mixed<false>(i, a.stride, a.pq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, xi4,
                                     ni4, tab, acc, true, a.has_bonds != 0);
             ti = __float_as_int(ni4.w);
             wi = __float_as_uint(xi4.w);
@@ -392,6 +406,14 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
         }
         fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
         if (a.f_ext) { fx += a.f_ext[3 * i]; fy += a.f_ext[3 * i + 1]; fz += a.f_ext[3 * i + 2]; }
+        if (a.push) {
+            // constrain_volume push along the particle's cell normal (src/constrain_volume.h:67-75)
+            const int kc = a.aff[i];
+            if (kc >= 0) {
+                const double sp = (*a.push) * c_ffd.mass[ti];
+                fx += sp * a.cell_normal[3 * kc]; fy += sp * a.cell_normal[3 * kc + 1]; fz += sp * a.cell_normal[3 * kc + 2];
+            }
+        }
         // rotation: t = n x t; o += dt t; n = normalize(n + (o x n) dt)
         const double nx = a.n_in[3 * i], ny = a.n_in[3 * i + 1], nz = a.n_in[3 * i + 2];
         double ox = a.o[3 * i], oy = a.o[3 * i + 1], oz = a.o[3 * i + 2];
@@ -426,9 +448,10 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin(StepArgs a) {
         const float r1 = __double2float_rn(py - a.anc_grid.hd * (double)(ai.y + a.anc_grid.A0[1]));
         const float r2 = __double2float_rn(pz - a.anc_grid.hd * (double)(ai.z + a.anc_grid.A0[2]));
         if (MIXED) {
-            a.xq_out[i] = make_float4(r0, r1, r2, __uint_as_float(wi));
-            a.nq_out[i] = make_float4(__double2float_rn(gx), __double2float_rn(gy), __double2float_rn(gz),
-                                      __int_as_float(ti));
+            PQ q;
+            q.x = make_float4(r0, r1, r2, __uint_as_float(wi));
+            q.n = make_float4(__double2float_rn(gx), __double2float_rn(gy), __double2float_rn(gz), __int_as_float(ti));
+            a.pq_out[i] = q;
         }
         const float e0 = r0 - a.rb[3 * i], e1 = r1 - a.rb[3 * i + 1], e2 = r2 - a.rb[3 * i + 2];
         d2 = e0 * e0 + e1 * e1 + e2 * e2;
@@ -458,9 +481,10 @@ __global__ __launch_bounds__(BLOCK) void k_step_nh_second(StepArgs a, double *f6
         int ti;
         Acc<double> acc;
         if constexpr (MIXED) {
-            accumulate_mixed<false>(i, a.stride, a.xq_in, a.nq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid,
-                                    a.xq_in[i], a.nq_in[i], tab, acc, true, a.has_bonds != 0);
-            ti = __float_as_int(a.nq_in[i].w);
+            const PQ qi = a.pq_in[i];
+            accumulate_mixed<false>(i, a.stride, a.pq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid,
+                                    qi.x, qi.n, tab, acc, true, a.has_bonds != 0);
+            ti = __float_as_int(qi.n.w);
         } else {
             accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
                                    tab, acc, true, a.has_bonds != 0);

@@ -65,15 +65,16 @@ __device__ inline void rotate_director(double *n, const double *o, double dt) {
 
 // refresh the FP32 mirrors of particle i from its FP64 state (build-time anchor)
 __device__ inline void write_mirror(const IntegArgs &a, long long i, const double *x, const double *n, int ty) {
-    if (!a.xq) return;
+    if (!a.pq) return;
     const uint32_t w = a.anc[i];
     const Anchor &g = a.anc_grid;
     const long long A[3] = {(long long)(w & g.mx) + g.A0[0], (long long)((w >> g.shy) & g.my) + g.A0[1],
                             (long long)(w >> g.shz) + g.A0[2]};
-    a.xq[i] = make_float4(__double2float_rn(x[0] - g.hd * (double)A[0]), __double2float_rn(x[1] - g.hd * (double)A[1]),
-                          __double2float_rn(x[2] - g.hd * (double)A[2]), __uint_as_float(w));
-    a.nq[i] = make_float4(__double2float_rn(n[0]), __double2float_rn(n[1]), __double2float_rn(n[2]),
-                          __int_as_float(ty));
+    PQ q;
+    q.x = make_float4(__double2float_rn(x[0] - g.hd * (double)A[0]), __double2float_rn(x[1] - g.hd * (double)A[1]),
+                      __double2float_rn(x[2] - g.hd * (double)A[2]), __uint_as_float(w));
+    q.n = make_float4(__double2float_rn(n[0]), __double2float_rn(n[1]), __double2float_rn(n[2]), __int_as_float(ty));
+    a.pq[i] = q;
 }
 
 __device__ inline void block_sum_store(double v, double *partial) {
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(IntegArgs a) {
         if (x < a.box_lo[d]) { a.x[3 * i + d] = a.box_lo[d] + (a.box_lo[d] - x); a.v[3 * i + d] = -a.v[3 * i + d]; }
         else if (x > a.box_hi[d]) { a.x[3 * i + d] = a.box_hi[d] - (x - a.box_hi[d]); a.v[3 * i + d] = -a.v[3 * i + d]; }
     }
-    if (a.xq) {
+    if (a.pq) {
         double x[3], n[3];
         for (int d = 0; d < 3; ++d) { x[d] = a.x[3 * i + d]; n[d] = a.nrm[3 * i + d]; }
         write_mirror(a, i, x, n, a.type[i]);
@@ -308,7 +309,8 @@ __global__ __launch_bounds__(BLOCK) void k_gather3f(long long n, const uint32_t 
     const long long s = slot[i];
     for (int d = 0; d < 3; ++d) dst[3 * i + d] = (float)src[3 * s + d];
 }
-__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t v, long long n, Gate gt) {
+    if (gate_closed(gt)) return;
     for (long long i = (long long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long long)gridDim.x * BLOCK) p[i] = v;
 }
 
@@ -388,11 +390,11 @@ void launch_gather_slots(long long n, const uint32_t *slot, const double *src3, 
 void launch_gather_slots_f(long long n, const uint32_t *slot, const double *src3, float *dst3, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_gather3f, dim3(nblk(n)), dim3(BLOCK), 0, s, n, slot, src3, dst3);
 }
-void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s) {
+void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s, Gate gt) {
     if (n <= 0) return;
     long long nb = nblk(n);
     if (nb > 65536) nb = 65536;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)nb), dim3(BLOCK), 0, s, p, v, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)nb), dim3(BLOCK), 0, s, p, v, n, gt);
 }
 
 }  // namespace orbc

@@ -51,7 +51,8 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t *total) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scan_tiles(const uint32_t *in, uint32_t *out, long long n,
-                                                      uint32_t *tilesums) {
+                                                      uint32_t *tilesums, Gate gt) {
+    if (gate_closed(gt)) return;
     const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
@@ -73,7 +74,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tiles(const uint32_t *in, uint32
 }
 
 // single block: exclusive scan of the tile sums in place (any length)
-__global__ __launch_bounds__(BLOCK) void k_scan_top(uint32_t *sums, long long m) {
+__global__ __launch_bounds__(BLOCK) void k_scan_top(uint32_t *sums, long long m, Gate gt) {
+    if (gate_closed(gt)) return;
     __shared__ uint32_t carry_s;
     if (threadIdx.x == 0) carry_s = 0;
     __syncthreads();
@@ -101,7 +103,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_top(uint32_t *sums, long long m)
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_add(uint32_t *out, long long n, const uint32_t *tilesums) {
+__global__ __launch_bounds__(BLOCK) void k_scan_add(uint32_t *out, long long n, const uint32_t *tilesums, Gate gt) {
+    if (gate_closed(gt)) return;
     const uint32_t add = tilesums[blockIdx.x];
     const long long base = (long long)blockIdx.x * SCAN_TILE;
     for (int k = threadIdx.x; k < SCAN_TILE; k += BLOCK) {
@@ -111,7 +114,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_add(uint32_t *out, long long n, 
 }
 
 __global__ __launch_bounds__(BLOCK) void k_rs_hist(const uint32_t *kin, long long n, int shift, uint32_t mask,
-                                                   uint32_t *hist, long long ntiles) {
+                                                   uint32_t *hist, long long ntiles, Gate gt) {
+    if (gate_closed(gt)) return;
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -127,7 +131,8 @@ __global__ __launch_bounds__(BLOCK) void k_rs_hist(const uint32_t *kin, long lon
 
 __global__ __launch_bounds__(BLOCK) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                       uint32_t *vout, long long n, int shift, int nbits,
-                                                      const uint32_t *offs, long long ntiles) {
+                                                      const uint32_t *offs, long long ntiles, Gate gt) {
+    if (gate_closed(gt)) return;
     __shared__ uint32_t base[256];
     __shared__ uint32_t wcnt[BLOCK / 64][256];
     const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
@@ -177,13 +182,13 @@ long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 
 size_t scan_temp_bytes(long long n) { return sizeof(uint32_t) * (size_t)(cdiv(n, SCAN_TILE) + 1); }
 
-void exclusive_scan_u32(const uint32_t *in, uint32_t *out, long long n, void *temp, hipStream_t s) {
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, long long n, void *temp, hipStream_t s, Gate gt) {
     if (n <= 0) return;
     const long long nt = cdiv(n, SCAN_TILE);
     uint32_t *tilesums = static_cast<uint32_t *>(temp);
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(BLOCK), 0, s, in, out, n, tilesums);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, s, tilesums, nt);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(BLOCK), 0, s, out, n, tilesums);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(BLOCK), 0, s, in, out, n, tilesums, gt);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, s, tilesums, nt, gt);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(BLOCK), 0, s, out, n, tilesums, gt);
 }
 
 size_t radix_temp_bytes(long long n) {
@@ -193,7 +198,7 @@ size_t radix_temp_bytes(long long n) {
 }
 
 void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, long long n,
-                      int bits, void *temp, hipStream_t s, bool *result_in_alt) {
+                      int bits, void *temp, hipStream_t s, bool *result_in_alt, Gate gt) {
     *result_in_alt = false;
     if (n <= 1 || bits <= 0) return;
     const long long nt = cdiv(n, RS_TILE);
@@ -206,10 +211,10 @@ void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
     for (int shift = 0; shift < bits; shift += 8) {
         const int nb = (bits - shift) < 8 ? (bits - shift) : 8;
         const uint32_t mask = (1u << nb) - 1u;
-        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nt), dim3(BLOCK), 0, s, ki, n, shift, mask, hist, nt);
-        exclusive_scan_u32(hist, offs, hn, stemp, s);
+        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nt), dim3(BLOCK), 0, s, ki, n, shift, mask, hist, nt, gt);
+        exclusive_scan_u32(hist, offs, hn, stemp, s, gt);
         hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nt), dim3(BLOCK), 0, s, ki, vi, ko, vo, n, shift, nb,
-                           offs, nt);
+                           offs, nt, gt);
         uint32_t *tk = ki, *tv = vi;
         ki = ko; vi = vo; ko = tk; vo = tv;
         alt = !alt;

@@ -55,25 +55,31 @@ __global__ __launch_bounds__(BLOCK) void k_morton_ref(const float *p, long long 
 }
 
 __global__ __launch_bounds__(BLOCK) void k_centroid_keys(const float *c, long long m, Grid g, uint32_t *keys,
-                                                         uint32_t *vals) {
+                                                         uint32_t *vals, Gate gt) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= m) return;
+    if (i >= m || gate_closed(gt)) return;
     keys[i] = (uint32_t)glin(gcoord(c[3 * i], g, 0), gcoord(c[3 * i + 1], g, 1), gcoord(c[3 * i + 2], g, 2), g);
     vals[i] = (uint32_t)i;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_cgrid_ranges(const uint32_t *k, long long m, uint32_t *start,
-                                                        uint32_t *end) {
+                                                        uint32_t *end, Gate gt) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= m) return;
+    if (i >= m || gate_closed(gt)) return;
     if (i == 0 || k[i] != k[i - 1]) start[k[i]] = (uint32_t)i;
     if (i + 1 == m || k[i] != k[i + 1]) end[k[i]] = (uint32_t)(i + 1);
 }
 
-__device__ inline bool voro_gated(const VoroArgs &a) { return a.gstop && (*a.gstop != 0 || *a.gslot != 0u); }
+// drift of the centroids since they were binned (0 when the index is current)
+__device__ inline float centroid_drift(const VoroArgs &a) {
+    if (!a.cdisp) return 0.f;
+    const float d2 = __uint_as_float(*a.cdisp);
+    return d2 > 0.f ? sqrtf(d2) * 1.0001f + 1e-5f : 0.f;
+}
 
 // nearest centroid (lowest index on equal squared distance), grid rings outward
 __device__ int nearest_one(const VoroArgs &a, float p0, float p1, float p2) {
+    const float drift = centroid_drift(a);
     const Grid &g = a.cg;
     const int cx = gcoord(p0, g, 0), cy = gcoord(p1, g, 1), cz = gcoord(p2, g, 2);
     float best = INFINITY;
@@ -100,9 +106,10 @@ __device__ int nearest_one(const VoroArgs &a, float p0, float p1, float p2) {
                 }
             }
         }
-        // every unvisited centroid is at least R cell edges away from the query
-        const float reach = (float)R * (float)g.cs;
-        if (bi >= 0 && best <= reach * reach) break;
+        // every unvisited centroid was binned at least R cell edges away from
+        // the query and has drifted by at most `drift` since
+        const float reach = (float)R * (float)g.cs - drift;
+        if (bi >= 0 && reach > 0.f && best <= reach * reach) break;
     }
     return bi;
 }
@@ -116,7 +123,8 @@ __device__ int nearest_warm(const VoroArgs &a, float p0, float p1, float p2, int
     const Grid &g = a.cg;
     float best = normsq_unfused(p0 - a.centroids[3 * k0], p1 - a.centroids[3 * k0 + 1], p2 - a.centroids[3 * k0 + 2]);
     int bi = k0;
-    const float r = sqrtf(best) * 1.0001f + 1e-4f;
+    // a centroid as near as k0 was binned within r + drift of p
+    const float r = sqrtf(best) * 1.0001f + 1e-4f + centroid_drift(a);
     const int x0 = gcoord(p0 - r, g, 0), x1 = gcoord(p0 + r, g, 0);
     const int y0 = gcoord(p1 - r, g, 1), y1 = gcoord(p1 + r, g, 1);
     const int z0 = gcoord(p2 - r, g, 2), z1 = gcoord(p2 + r, g, 2);
@@ -138,7 +146,7 @@ __device__ int nearest_warm(const VoroArgs &a, float p0, float p1, float p2, int
 __global__ __launch_bounds__(BLOCK) void k_nearest(VoroArgs a, const float *q, long long nq, const double *qd,
                                                    int32_t *out, int warm) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= nq || voro_gated(a)) return;
+    if (i >= nq || gate_closed(a.gate)) return;
     float p0, p1, p2;
     if (qd) {
         p0 = (float)qd[3 * i]; p1 = (float)qd[3 * i + 1]; p2 = (float)qd[3 * i + 2];
@@ -177,13 +185,21 @@ __global__ __launch_bounds__(BLOCK) void k_within(VoroArgs a, const float *q, lo
     counts[i] = cnt;
 }
 
-// Segmented sum over runs of equal keys in consecutive lanes (particles are
-// Morton-ordered and cells compact, so a wave holds a few runs): the last lane
-// of each run adds the run's totals with one atomic per component.  Lanes with
-// key < 0 contribute nothing.  Integer sums are exact, so the grouping does not
-// change the result.
-__device__ inline void wave_run_add4(int key, unsigned long long v0, unsigned long long v1, unsigned long long v2,
-                                     unsigned long long v3, unsigned long long *sums) {
+// Per-cell fixed-point sums of a lipid field (positions or directors) plus the
+// lipid count: sums[4k..4k+2] += round(field * scale), sums[4k+3] += 1.
+//
+// Device-scope 64-bit atomics are slow, and the cells (~14 lipids) do not
+// follow the particle order (Morton order of the binning grid), so the sums are
+// aggregated first: within a wave over runs of equal cell in consecutive lanes,
+// then per block (SUM_CHUNK particles) in an LDS hash table keyed by cell, and
+// only the table is flushed with global atomics.  A key that finds the table
+// full goes straight to the global atomics.  Integer sums are exact, so the
+// grouping does not change the result.
+constexpr int SUM_CHUNK = 2048;      // particles per block
+constexpr int SUM_SLOTS = 512;       // LDS hash slots (power of two)
+
+// inclusive segmented scan over runs of equal keys; true on the last lane of a run
+__device__ inline bool wave_run_reduce4(int key, unsigned long long v[4]) {
     const int lane = __lane_id();
     const int prev = __shfl_up(key, 1, 64), next = __shfl_down(key, 1, 64);
     const bool head = lane == 0 || prev != key;
@@ -192,55 +208,96 @@ __device__ inline void wave_run_add4(int key, unsigned long long v0, unsigned lo
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
     const int start = 63 - __clzll((long long)(heads & upto));      // first lane of my run
     for (int off = 1; off < 64; off <<= 1) {
-        const unsigned long long a0 = __shfl_up(v0, off, 64), a1 = __shfl_up(v1, off, 64);
-        const unsigned long long a2 = __shfl_up(v2, off, 64), a3 = __shfl_up(v3, off, 64);
-        if (lane - off >= start) { v0 += a0; v1 += a1; v2 += a2; v3 += a3; }
+        unsigned long long t[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) t[c] = __shfl_up(v[c], off, 64);
+        if (lane - off >= start)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] += t[c];
     }
-    if (tail && key >= 0) {
-        atomicAdd(&sums[4 * key], v0);
-        atomicAdd(&sums[4 * key + 1], v1);
-        atomicAdd(&sums[4 * key + 2], v2);
-        atomicAdd(&sums[4 * key + 3], v3);
-    }
+    return tail;
 }
 
-// fixed-point per-cell sums of lipid positions: sums[4k..4k+2] = sum x, [4k+3] = count
-__global__ __launch_bounds__(BLOCK) void k_centroid_sums(VoroArgs a, unsigned long long *sums) {
-    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    int k = -1;
-    unsigned long long v[3] = {0, 0, 0};
-    if (i < a.n && a.type[i] == 0) {
-        k = a.aff[i];
+template <bool DIRECTORS>
+__global__ __launch_bounds__(BLOCK) void k_cell_sums(VoroArgs a, unsigned long long *sums) {
+    if (gate_closed(a.gate)) return;
+    __shared__ int tkey[SUM_SLOTS];
+    __shared__ unsigned long long tval[SUM_SLOTS][4];
+    for (int t = threadIdx.x; t < SUM_SLOTS; t += BLOCK) {
+        tkey[t] = -1;
+        tval[t][0] = tval[t][1] = tval[t][2] = tval[t][3] = 0ull;
+    }
+    __syncthreads();
+    const double *src = DIRECTORS ? a.nrm : a.x;
+    const double scale = DIRECTORS ? kNrmScale : kPosScale;
+    const long long base = (long long)blockIdx.x * SUM_CHUNK;
+    for (int it = 0; it < SUM_CHUNK / BLOCK; ++it) {
+        const long long i = base + (long long)it * BLOCK + threadIdx.x;
+        int k = -1;
+        unsigned long long v[4] = {0ull, 0ull, 0ull, 0ull};
+        if (i < a.n && a.type[i] == 0) {
+            k = a.aff[i];
+            if (k >= 0) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) v[d] = (unsigned long long)(long long)llrint(src[3 * i + d] * scale);
+                v[3] = 1ull;
+            }
+        }
+        const bool tail = wave_run_reduce4(k, v);
+        if (tail && k >= 0) {
+            unsigned h = ((unsigned)k * 2654435761u) >> (32 - 9);     // 9 = log2(SUM_SLOTS)
+            bool placed = false;
+            for (int probe = 0; probe < SUM_SLOTS; ++probe) {
+                const int prev = atomicCAS(&tkey[h], -1, k);
+                if (prev == -1 || prev == k) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) atomicAdd(&tval[h][c], v[c]);
+                    placed = true;
+                    break;
+                }
+                h = (h + 1) & (SUM_SLOTS - 1);
+            }
+            if (!placed)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) atomicAdd(&sums[4 * k + c], v[c]);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < SUM_SLOTS; t += BLOCK) {
+        const int k = tkey[t];
         if (k >= 0)
-            for (int d = 0; d < 3; ++d) v[d] = (unsigned long long)(long long)llrint(a.x[3 * i + d] * kPosScale);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) atomicAdd(&sums[4 * k + c], tval[t][c]);
     }
-    wave_run_add4(k, v[0], v[1], v[2], k >= 0 ? 1ull : 0ull, sums);
 }
 
-// update_centroid, src/voronoi.h:124-140 (empty cells keep their centroid)
-__global__ __launch_bounds__(BLOCK) void k_centroid_finish(VoroArgs a, const unsigned long long *sums) {
+// update_centroid, src/voronoi.h:124-140 (empty cells keep their centroid).
+// The sums are cleared after reading: they are zero whenever no kernel pair
+// (sums, finish) is in flight, so no fill is needed before the next sums.
+__global__ __launch_bounds__(BLOCK) void k_centroid_finish(VoroArgs a, unsigned long long *sums) {
+    if (gate_closed(a.gate)) return;
     const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= a.ncell || voro_gated(a)) return;
     float *c = a.centroids;
-    const long long cnt = (long long)sums[4 * k + 3];
-    if (cnt == 0) return;
-    for (int d = 0; d < 3; ++d) {
-        const double s = (double)(long long)sums[4 * k + d] / kPosScale;
-        c[3 * k + d] = (float)(s / (double)cnt);
+    float d2 = 0.f;
+    if (k < a.ncell) {
+        unsigned long long q[4];
+        for (int d = 0; d < 4; ++d) { q[d] = sums[4 * k + d]; sums[4 * k + d] = 0ull; }
+        const long long cnt = (long long)q[3];
+        if (cnt > 0) {
+            for (int d = 0; d < 3; ++d) {
+                const double s = (double)(long long)q[d] / kPosScale;
+                c[3 * k + d] = (float)(s / (double)cnt);
+            }
+        }
     }
-}
-
-// sum of lipid directors and lipid count per cell, src/constrain_volume.h:55
-__global__ __launch_bounds__(BLOCK) void k_normal_sums(VoroArgs a, unsigned long long *sums) {
-    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    int k = -1;
-    unsigned long long v[3] = {0, 0, 0};
-    if (i < a.n && a.type[i] == 0) {
-        k = a.aff[i];
-        if (k >= 0)
-            for (int d = 0; d < 3; ++d) v[d] = (unsigned long long)(long long)llrint(a.nrm[3 * i + d] * kNrmScale);
+    if (a.cdisp && k < a.ncell) {
+        const float e0 = c[3 * k] - a.cref[3 * k], e1 = c[3 * k + 1] - a.cref[3 * k + 1], e2 = c[3 * k + 2] - a.cref[3 * k + 2];
+        d2 = e0 * e0 + e1 * e1 + e2 * e2;
     }
-    wave_run_add4(k, v[0], v[1], v[2], k >= 0 ? 1ull : 0ull, sums);
+    for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off, 64));
+    if (a.cdisp && __lane_id() == 0 && d2 > 0.f &&
+        __float_as_uint(d2) > __hip_atomic_load(a.cdisp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(a.cdisp, __float_as_uint(d2));
 }
 
 __device__ inline void block_sum_store(double v, double *partial) {
@@ -256,20 +313,37 @@ __device__ inline void block_sum_store(double v, double *partial) {
     }
 }
 
-// per-cell normal, orientation and cone volume, src/constrain_volume.h:54-62
-__global__ __launch_bounds__(BLOCK) void k_volume_finish(const unsigned long long *sums, const float *c, long long m,
-                                                         const double *center, double *normals, double *partial) {
+// per-cell normal, orientation and cone volume, src/constrain_volume.h:54-62.
+// The centre of the centroids is reduced by every block from the per-block
+// partials of k_centroid_center (3 x ncp); the cell sums are cleared after
+// reading (see k_centroid_finish).  Volume partials go to vol_partial.
+__global__ __launch_bounds__(BLOCK) void k_volume_finish(unsigned long long *sums, const float *c, long long m,
+                                                         const double *center_partial, long long ncp,
+                                                         double *normals, double *vol_partial, Gate gt) {
+    if (gate_closed(gt)) return;
+    __shared__ double s_center[3];
+    if (threadIdx.x < 64) {
+        for (int d = 0; d < 3; ++d) {
+            double v = 0;
+            for (long long b = threadIdx.x; b < ncp; b += 64) v += center_partial[d * ncp + b];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (threadIdx.x == 0) s_center[d] = v / (double)m;
+        }
+    }
+    __syncthreads();
     const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
     double vol = 0.0;
     if (k < m) {
-        const long long cnt = (long long)sums[4 * k + 3];
-        double nx = (double)(long long)sums[4 * k] / kNrmScale;
-        double ny = (double)(long long)sums[4 * k + 1] / kNrmScale;
-        double nz = (double)(long long)sums[4 * k + 2] / kNrmScale;
+        unsigned long long q[4];
+        for (int d = 0; d < 4; ++d) { q[d] = sums[4 * k + d]; sums[4 * k + d] = 0ull; }
+        const long long cnt = (long long)q[3];
+        double nx = (double)(long long)q[0] / kNrmScale;
+        double ny = (double)(long long)q[1] / kNrmScale;
+        double nz = (double)(long long)q[2] / kNrmScale;
         const double nn = sqrt(nx * nx + ny * ny + nz * nz);
         if (cnt > 0 && nn > 0) {
             nx /= nn; ny /= nn; nz /= nn;
-            const double dx = c[3 * k] - center[0], dy = c[3 * k + 1] - center[1], dz = c[3 * k + 2] - center[2];
+            const double dx = c[3 * k] - s_center[0], dy = c[3 * k + 1] - s_center[1], dz = c[3 * k + 2] - s_center[2];
             if (nx * dx + ny * dy + nz * dz < 0) { nx = -nx; ny = -ny; nz = -nz; }
             const double h = dx * nx + dy * ny + dz * nz;
             vol = h * (double)cnt * 3.1415926 * 1.26 / 4.0 / 3.0 * 1e-6;
@@ -278,10 +352,11 @@ __global__ __launch_bounds__(BLOCK) void k_volume_finish(const unsigned long lon
         }
         normals[3 * k] = nx; normals[3 * k + 1] = ny; normals[3 * k + 2] = nz;
     }
-    block_sum_store(vol, partial);
+    block_sum_store(vol, vol_partial);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_centroid_center(const float *c, long long m, double *partial) {
+__global__ __launch_bounds__(BLOCK) void k_centroid_center(const float *c, long long m, double *partial, Gate gt) {
+    if (gate_closed(gt)) return;
     const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
     for (int d = 0; d < 3; ++d) {
         block_sum_store(k < m ? (double)c[3 * k + d] : 0.0, partial + d * gridDim.x);
@@ -294,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) void k_centroid_center(const float *c, long 
 __global__ __launch_bounds__(BLOCK) void k_volume_push(VoroArgs a, const double *normals, const double *fpush,
                                                        double *f_ext, double *f64, float *f32) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n || voro_gated(a)) return;
+    if (i >= a.n || gate_closed(a.gate)) return;
     const int k = a.aff[i];
     double add[3] = {0, 0, 0};
     if (k >= 0) {
@@ -314,7 +389,9 @@ __device__ inline double wave_sum_d(double v) {
 }
 
 // one wavefront: V = sum of the block partials, push = strength (target - V) / target
-__global__ void k_volume_force(const double *vol_partial, long long nb, double target, double strength, double *scal) {
+__global__ void k_volume_force(const double *vol_partial, long long nb, double target, double strength, double *scal,
+                               Gate gt) {
+    if (gate_closed(gt)) return;
     double v = 0;
     for (long long b = threadIdx.x; b < nb; b += 64) v += vol_partial[b];
     v = wave_sum_d(v);
@@ -324,28 +401,20 @@ __global__ void k_volume_force(const double *vol_partial, long long nb, double t
     }
 }
 
-// one wavefront: centre of the centroids
-__global__ void k_center_finish(const double *partial, long long nb, long long m, double *center) {
-    for (int d = 0; d < 3; ++d) {
-        double v = 0;
-        for (long long b = threadIdx.x; b < nb; b += 64) v += partial[d * nb + b];
-        v = wave_sum_d(v);
-        if (threadIdx.x == 0) center[d] = v / (double)m;
-    }
-}
-
 }  // namespace
 
 void launch_morton_ref(const float *pts, long long n, double bsize, uint32_t *keys, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_morton_ref, dim3(nblk(n)), dim3(BLOCK), 0, s, pts, n, bsize, keys);
 }
 
-void launch_centroid_keys(const float *c, long long m, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s) {
-    if (m > 0) hipLaunchKernelGGL(k_centroid_keys, dim3(nblk(m)), dim3(BLOCK), 0, s, c, m, g, keys, vals);
+void launch_centroid_keys(const float *c, long long m, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s,
+                          Gate gt) {
+    if (m > 0) hipLaunchKernelGGL(k_centroid_keys, dim3(nblk(m)), dim3(BLOCK), 0, s, c, m, g, keys, vals, gt);
 }
 
-void launch_cgrid_ranges(const uint32_t *sorted_keys, long long m, uint32_t *start, uint32_t *end, hipStream_t s) {
-    if (m > 0) hipLaunchKernelGGL(k_cgrid_ranges, dim3(nblk(m)), dim3(BLOCK), 0, s, sorted_keys, m, start, end);
+void launch_cgrid_ranges(const uint32_t *sorted_keys, long long m, uint32_t *start, uint32_t *end, hipStream_t s,
+                         Gate gt) {
+    if (m > 0) hipLaunchKernelGGL(k_cgrid_ranges, dim3(nblk(m)), dim3(BLOCK), 0, s, sorted_keys, m, start, end, gt);
 }
 
 void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out, bool warm,
@@ -358,26 +427,35 @@ void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, 
     if (nq > 0) hipLaunchKernelGGL(k_within, dim3(nblk(nq)), dim3(BLOCK), 0, s, a, q, nq, rmax, cap, counts, out);
 }
 
+// sums must be zero on entry (they are cleared by the finish kernels)
 void launch_centroid_sums(const VoroArgs &a, unsigned long long *sums, hipStream_t s) {
-    launch_fill_u32(reinterpret_cast<uint32_t *>(sums), 0u, 8 * a.ncell, s);
-    if (a.n > 0) hipLaunchKernelGGL(k_centroid_sums, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a, sums);
+    if (a.n > 0)
+        hipLaunchKernelGGL(k_cell_sums<false>, dim3((unsigned)((a.n + SUM_CHUNK - 1) / SUM_CHUNK)), dim3(BLOCK), 0, s,
+                           a, sums);
 }
 
-void launch_centroid_finish(const VoroArgs &a, const unsigned long long *sums, hipStream_t s) {
+void launch_centroid_finish(const VoroArgs &a, unsigned long long *sums, hipStream_t s) {
     if (a.ncell > 0) hipLaunchKernelGGL(k_centroid_finish, dim3(nblk(a.ncell)), dim3(BLOCK), 0, s, a, sums);
+}
+
+// centre-of-centroids partials (3 x nblk(ncell)) at partial; rerun whenever the centroids change
+void launch_centroid_center(const VoroArgs &a, double *partial, hipStream_t s) {
+    if (a.ncell > 0)
+        hipLaunchKernelGGL(k_centroid_center, dim3((unsigned)nblk(a.ncell)), dim3(BLOCK), 0, s, a.centroids, a.ncell,
+                           partial, a.gate);
 }
 
 void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals, double *partial, double *scal,
                    double target, double strength, hipStream_t s) {
-    // scal: [0] V, [1] push, [2..4] center; partial >= 3 * nblk(ncell) doubles
+    // scal: [0] V, [1] push; partial: centre partials (launch_centroid_center), then volume partials
     const long long nbc = nblk(a.ncell);
-    hipLaunchKernelGGL(k_centroid_center, dim3((unsigned)nbc), dim3(BLOCK), 0, s, a.centroids, a.ncell, partial);
-    hipLaunchKernelGGL(k_center_finish, dim3(1), dim3(64), 0, s, partial, nbc, a.ncell, scal + 2);
-    launch_fill_u32(reinterpret_cast<uint32_t *>(sums), 0u, 8 * a.ncell, s);
-    if (a.n > 0) hipLaunchKernelGGL(k_normal_sums, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a, sums);
-    hipLaunchKernelGGL(k_volume_finish, dim3((unsigned)nbc), dim3(BLOCK), 0, s, sums, a.centroids, a.ncell, scal + 2,
-                       normals, partial);
-    hipLaunchKernelGGL(k_volume_force, dim3(1), dim3(64), 0, s, partial, nbc, target, strength, scal);
+    if (a.n > 0)
+        hipLaunchKernelGGL(k_cell_sums<true>, dim3((unsigned)((a.n + SUM_CHUNK - 1) / SUM_CHUNK)), dim3(BLOCK), 0, s,
+                           a, sums);
+    hipLaunchKernelGGL(k_volume_finish, dim3((unsigned)nbc), dim3(BLOCK), 0, s, sums, a.centroids, a.ncell, partial,
+                       nbc, normals, partial + 3 * nbc, a.gate);
+    hipLaunchKernelGGL(k_volume_force, dim3(1), dim3(64), 0, s, partial + 3 * nbc, nbc, target, strength, scal,
+                       a.gate);
 }
 
 void launch_volume_push(const VoroArgs &a, const double *normals, const double *fpush, double *f_ext, double *f64,

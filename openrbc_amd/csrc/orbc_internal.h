@@ -62,6 +62,23 @@ struct Box {
     int per[3];
 };
 
+// Skip gate for work launched ahead inside a batched run (orbc_run).  The
+// work of step k is enqueued before step k's fused kernel; when *stop is set or
+// step k-1 raised its skin flag (*slot), step k will be skipped, and every
+// kernel of its work returns at once.  A default Gate is always open.
+struct Gate {
+    const int *stop = nullptr;
+    const unsigned *slot = nullptr;
+};
+__device__ inline bool gate_closed(const Gate &g) { return g.stop && (*g.stop != 0 || *g.slot != 0u); }
+
+// FP32 mirror of one particle for the mixed path, one 32-byte record so a
+// neighbour gather touches one cache line: x = position offset from the
+// anchor (xyz) + packed anchor (w, bits); n = director (xyz) + type (w, bits).
+struct PQ {
+    float4 x, n;
+};
+
 // Everything a step kernel reads or writes, by value in kernel arguments.
 struct StepArgs {
     long long n;
@@ -70,8 +87,8 @@ struct StepArgs {
     const double *x_in, *n_in;
     double *x_out, *n_out, *v, *o;
     // FP32 mirrors (mixed precision)
-    const float4 *xq_in, *nq_in;
-    float4 *xq_out, *nq_out;
+    const PQ *pq_in;
+    PQ *pq_out;
     const uint8_t *type;
     const uint32_t *id;
     const uint32_t *anc;       // packed anchors (FP64 path)
@@ -80,7 +97,11 @@ struct StepArgs {
     const uint32_t *bnd;
     const uint8_t *nbnd;
     int has_bonds;
-    const double *f_ext;       // extra per-particle force (volume constraint), or null
+    const double *f_ext;       // extra per-particle force, or null
+    // volume push fused into the step (constrain_volume): f += *push * mass_type * normal[aff]
+    const double *push;        // device scalar strength (V0 - V) / V0, or null
+    const double *cell_normal; // 3 per Voronoi cell
+    const int32_t *aff;        // Voronoi cell of each particle
     Anchor anc_grid;
     double dt;
     double gamma[NT], sigma[NT];
@@ -102,7 +123,7 @@ struct StepArgs {
 struct ForceArgs {
     long long n, stride;
     const double *x, *nrm;
-    const float4 *xq, *nq;
+    const PQ *pq;
     const uint8_t *type;
     const uint32_t *id;
     const uint32_t *nbr, *nnbr;
@@ -121,10 +142,10 @@ void upload_ff(const FFTab<double> &d, const FFTab<float> &f, hipStream_t s);
 
 // k_sort.hip
 size_t scan_temp_bytes(long long n);
-void exclusive_scan_u32(const uint32_t *in, uint32_t *out, long long n, void *temp, hipStream_t s);
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, long long n, void *temp, hipStream_t s, Gate gt = {});
 size_t radix_temp_bytes(long long n);
 void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, long long n,
-                      int bits, void *temp, hipStream_t s, bool *result_in_alt);
+                      int bits, void *temp, hipStream_t s, bool *result_in_alt, Gate gt = {});
 
 // k_build.hip
 void launch_bbox(const double *x, long long n, double *out6, hipStream_t s);
@@ -142,12 +163,12 @@ struct PermuteArgs {
 void launch_permute(const PermuteArgs &a, hipStream_t s);
 void launch_inverse(const uint32_t *slot, uint32_t *inv, long long n, hipStream_t s);
 void launch_refresh(long long n, const double *x, const double *nrm, const uint8_t *type, Anchor a,
-                    float4 *xq, float4 *nq, float *rb, uint32_t *anc, hipStream_t s);
+                    PQ *pq, float *rb, uint32_t *anc, hipStream_t s);
 void launch_cell_ranges(const double *x, long long n, Grid g, uint32_t *cstart, uint32_t *cend, hipStream_t s);
 struct NlistArgs {
     long long n, stride;
     const double *x;
-    const float4 *xq;        // FP32 mirrors with anchors (refreshed just before)
+    const PQ *pq;            // FP32 mirrors with anchors (refreshed just before)
     Anchor anc;
     const uint8_t *type;
     Grid g;
@@ -188,7 +209,7 @@ struct IntegArgs {
     const uint8_t *type;
     const uint32_t *id;
     const uint32_t *anc;
-    float4 *xq, *nq;   // mirrors refreshed in place when nonnull
+    PQ *pq;            // mirrors refreshed in place when nonnull
     Anchor anc_grid;
     double dt;
     double gamma[NT], sigma[NT];
@@ -212,7 +233,7 @@ void launch_scatter_slots_f(long long n, const uint32_t *slot, const float *src3
 void launch_scatter_slots1(long long n, const uint32_t *slot, const double *src, double *dst, hipStream_t s);
 void launch_gather_slots(long long n, const uint32_t *slot, const double *src3, double *dst3, hipStream_t s);
 void launch_gather_slots_f(long long n, const uint32_t *slot, const double *src3, float *dst3, hipStream_t s);
-void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s, Gate gt = {});
 
 // k_voronoi.hip
 struct VoroArgs {
@@ -227,22 +248,26 @@ struct VoroArgs {
     // centroid grid (for nearest / within queries)
     Grid cg;
     const uint32_t *cg_start, *cg_end, *cg_idx;
-    // inside a batched run: the kernels that write persistent state (centroids,
-    // affiliation, push force) do nothing when the step they precede is skipped,
-    // i.e. when *gstop is set or the previous step raised its skin flag *gslot
-    const int *gstop;
-    const unsigned *gslot;
+    Gate gate;               // inside a batched run: the step this work precedes may be skipped
+    // The centroid grid is built at neighbour rebuilds only: cref holds the
+    // centroids as binned, *cdisp the largest squared drift of a centroid since
+    // (as float bits).  Queries widen their reach by that drift, so they stay
+    // exact.  cdisp null: the index is current.
+    float *cref;
+    unsigned *cdisp;
 };
 void launch_morton_ref(const float *pts, long long n, double bsize, uint32_t *keys, hipStream_t s);
-void launch_centroid_keys(const float *c, long long ncell, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s);
+void launch_centroid_keys(const float *c, long long ncell, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s,
+                          Gate gt = {});
 void launch_cgrid_ranges(const uint32_t *sorted_keys, long long ncell, uint32_t *start, uint32_t *end,
-                         hipStream_t s);
+                         hipStream_t s, Gate gt = {});
 void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out, bool warm,
                     hipStream_t s);
 void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, int cap, int *counts,
                    int *out, hipStream_t s);
 void launch_centroid_sums(const VoroArgs &a, unsigned long long *sums, hipStream_t s);
-void launch_centroid_finish(const VoroArgs &a, const unsigned long long *sums, hipStream_t s);
+void launch_centroid_finish(const VoroArgs &a, unsigned long long *sums, hipStream_t s);
+void launch_centroid_center(const VoroArgs &a, double *partial, hipStream_t s);
 void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals, double *partial, double *scal,
                    double target, double strength, hipStream_t s);
 void launch_volume_push(const VoroArgs &a, const double *normals, const double *fpush, double *f_ext,

@@ -44,13 +44,13 @@ struct PointIndex {
 };
 
 void build_index(orbc_ctx *c, const float *pts, long long m, const Grid &g, uint32_t *k, uint32_t *v, uint32_t *k2,
-                 uint32_t *v2, void *temp, uint32_t *start, uint32_t *end, uint32_t **idx_out) {
-    launch_fill_u32(start, 0u, g.ncell, c->stream);
-    launch_fill_u32(end, 0u, g.ncell, c->stream);
-    launch_centroid_keys(pts, m, g, k, v, c->stream);
+                 uint32_t *v2, void *temp, uint32_t *start, uint32_t *end, uint32_t **idx_out, Gate gt = {}) {
+    launch_fill_u32(start, 0u, g.ncell, c->stream, gt);
+    launch_fill_u32(end, 0u, g.ncell, c->stream, gt);
+    launch_centroid_keys(pts, m, g, k, v, c->stream, gt);
     bool alt = false;
-    radix_sort_pairs(k, v, k2, v2, m, ceil_log2(g.ncell), temp, c->stream, &alt);
-    launch_cgrid_ranges(alt ? k2 : k, m, start, end, c->stream);
+    radix_sort_pairs(k, v, k2, v2, m, ceil_log2(g.ncell), temp, c->stream, &alt, gt);
+    launch_cgrid_ranges(alt ? k2 : k, m, start, end, c->stream, gt);
     *idx_out = alt ? v2 : v;
 }
 
@@ -68,8 +68,9 @@ VoroArgs voro_args(orbc_ctx *c) {
     a.cg_start = c->voro.cg_start;
     a.cg_end = c->voro.cg_end;
     a.cg_idx = c->voro.cg_idx;
-    a.gstop = c->voro.gstop;
-    a.gslot = c->voro.gslot;
+    a.gate = c->voro.gate;
+    a.cref = c->voro.cref;
+    a.cdisp = c->voro.cdisp;
     return a;
 }
 
@@ -88,8 +89,11 @@ void index_centroids(orbc_ctx *c) {
         V.cg_end = c->mem.alloc<uint32_t>(V.cg_cap);
     }
     uint32_t *idx = nullptr;
-    build_index(c, V.c, V.ncell, V.cg, V.ck, V.cv, V.ck2, V.cv2, V.temp, V.cg_start, V.cg_end, &idx);
+    build_index(c, V.c, V.ncell, V.cg, V.ck, V.cv, V.ck2, V.cv2, V.temp, V.cg_start, V.cg_end, &idx, V.gate);
     V.cg_idx = idx;
+    // binned positions and zero drift
+    ORBC_HIP_CHECK(hipMemcpyAsync(V.cref, V.c, 12 * V.ncell, hipMemcpyDeviceToDevice, c->stream));
+    launch_fill_u32(V.cdisp, 0u, 1, c->stream);
 }
 
 void partition_all(orbc_ctx *c) {
@@ -102,6 +106,7 @@ void update_centroids(orbc_ctx *c) {
     VoroArgs a = voro_args(c);
     launch_centroid_sums(a, c->voro.sums, c->stream);
     launch_centroid_finish(a, c->voro.sums, c->stream);
+    launch_centroid_center(a, c->voro.partial, c->stream);
 }
 
 void voro_alloc(orbc_ctx *c, long long ncell) {
@@ -113,8 +118,12 @@ void voro_alloc(orbc_ctx *c, long long ncell) {
     V.aff_alt = c->mem.alloc<int32_t>(c->n);
     V.c = c->mem.alloc<float>(3 * ncell);
     V.sums = c->mem.alloc<unsigned long long>(4 * ncell);
+    // the cell sums are kept at zero between uses (the finish kernels clear them)
+    ORBC_HIP_CHECK(hipMemsetAsync(V.sums, 0, sizeof(unsigned long long) * 4 * (size_t)ncell, c->stream));
     V.normals = c->mem.alloc<double>(3 * ncell);
-    V.partial = c->mem.alloc<double>(3 * (ncell / BLOCK + 2) + 16);
+    V.cref = c->mem.alloc<float>(3 * ncell);
+    V.cdisp = c->mem.alloc<unsigned>(1);
+    V.partial = c->mem.alloc<double>(4 * (ncell / BLOCK + 2) + 16);
     V.ck = c->mem.alloc<uint32_t>(ncell); V.cv = c->mem.alloc<uint32_t>(ncell);
     V.ck2 = c->mem.alloc<uint32_t>(ncell); V.cv2 = c->mem.alloc<uint32_t>(ncell);
     V.temp = c->mem.alloc<char>(radix_temp_bytes(ncell));
@@ -129,7 +138,7 @@ namespace orbc {
 void voronoi_free(orbc_ctx *c) {
     auto &V = c->voro;
     void *ps[] = {V.aff, V.aff_alt, V.c, V.sums, V.normals, V.partial, V.ck, V.cv, V.ck2, V.cv2, V.cg_start, V.cg_end,
-                  V.temp};
+                  V.temp, V.cref, V.cdisp};
     for (void *p : ps) c->mem.release(p);
     V = orbc_ctx::Voro{};
 }
@@ -141,16 +150,21 @@ void voronoi_rebuild_index(orbc_ctx *c) {
 // VoronoiDiagram::update + VCellList::update of both containers
 void voronoi_update_dev(orbc_ctx *c) {
     update_centroids(c);
-    index_centroids(c);
     partition_all(c);
     c->voro.updates++;
 }
 
-// cone-sum volume and push force into f_ext (and/or the f accumulators)
+// cone-sum volume, cell normals and push magnitude (scal + 8: V, push)
+void volume_eval(orbc_ctx *c, double target, double strength) {
+    VoroArgs a = voro_args(c);
+    launch_volume(a, c->voro.sums, c->voro.normals, c->voro.partial, c->scal + 8, target, strength, c->stream);
+}
+
+// ... and the push force into f_ext (and/or the f accumulators)
 double volume_push(orbc_ctx *c, double target, double strength, double *f_ext, bool to_f, bool read_back) {
     VoroArgs a = voro_args(c);
     double *scal = c->scal + 8;
-    launch_volume(a, c->voro.sums, c->voro.normals, c->voro.partial, scal, target, strength, c->stream);
+    volume_eval(c, target, strength);
     launch_volume_push(a, c->voro.normals, scal + 1, f_ext, to_f && !c->mixed() ? c->f64 : nullptr,
                        to_f && c->mixed() ? c->f32 : nullptr, c->stream);
     if (!read_back) return 0.0;
@@ -190,6 +204,7 @@ int orbc_voronoi_init(orbc_ctx *c, long long n_cells, int n_iter) {
         }
         index_centroids(c);
         partition_all(c);
+        launch_centroid_center(voro_args(c), c->voro.partial, c->stream);
         sync(c);
         if (c->cfg.neighbor_mode == ORBC_NEIGHBOR_VORONOI) rebuild(c);
         return (int)ORBC_OK;
@@ -215,6 +230,7 @@ int orbc_voronoi_set_centroids(orbc_ctx *c, long long n_cells, const float *cen)
         ORBC_HIP_CHECK(hipMemcpyAsync(c->voro.c, cen, 12 * n_cells, hipMemcpyHostToDevice, c->stream));
         index_centroids(c);
         partition_all(c);
+        launch_centroid_center(voro_args(c), c->voro.partial, c->stream);
         sync(c);
         if (c->cfg.neighbor_mode == ORBC_NEIGHBOR_VORONOI) rebuild(c);
         return (int)ORBC_OK;

@@ -441,3 +441,19 @@ def test_run_volume_needs_cells():
     orbc.system_to_simulation(sphere, sim)
     with pytest.raises(orbc.OrbcError):
         sim.run(2)
+
+
+def test_partition_exact_with_drifted_centroid_index(oracle):
+    """The centroid grid is rebuilt only at neighbour rebuilds; between them
+    the nearest-centroid search widens its reach by the measured centroid
+    drift.  After many Lloyd updates with a large skin (few rebuilds) the
+    affiliation must still equal the brute-force nearest centroid."""
+    sphere = orbc.random_sphere(r=14.0, seed=5)
+    sim = orbc.Simulation(precision=orbc.FP64, seed=3, skin=1.5, kBT=0.22, volume_target=-1.0, volume_strength=0.5)
+    orbc.system_to_simulation(sphere, sim, kT=0.22, tilt=0.02, seed=4)
+    sim.voronoi_init(0, 2)                    # few Lloyd steps: the centroids keep moving
+    sim.run(60)
+    sim.voronoi_update()
+    c, al, _ = sim.voronoi_get()
+    x = sim.get_state(orbc.LIPID)["x"]
+    assert np.array_equal(al, oracle.partition(x, c))
