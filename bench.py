@@ -191,7 +191,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--minimize", type=int, default=1000)
-    ap.add_argument("--skin", type=float, default=0.4)
+    ap.add_argument("--skin", type=float, default=0.6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-lloyd", type=int, default=8)

@@ -84,7 +84,8 @@ typedef struct orbc_config {
     double Q;                   /* Nose-Hoover mass; 0 => 0.01 * N (src/runtime_parameter.h:76) */
     double zeta;                /* Nose-Hoover friction, 0 */
     double box_lo[3], box_hi[3];/* bounce-back box, +-1000 (src/runtime_parameter.h:111-115) */
-    int periodic[3];            /* 0: bounce/open, 1: periodic (no reference counterpart) */
+    int periodic[3];            /* 0: bounce/open, 1: periodic over [box_lo, box_hi) (minimum image;
+                                   positions wrapped at neighbour rebuilds; no reference counterpart) */
     double skin;                /* Verlet skin, 0.4 */
     int rebuild_period;         /* 0: rebuild on the skin criterion; k>0: every k steps */
     int neighbor_mode;          /* ORBC_NEIGHBOR_* */
@@ -107,8 +108,12 @@ typedef struct orbc_observables {
     long long step;     /* completed steps */
     long long rebuilds; /* neighbour rebuilds so far */
     long long pairs;    /* directed neighbour-list entries at the last rebuild */
-    double anchor_h;    /* spacing of the FP32 anchor grid (mixed geometry) */
+    double anchor_h;    /* base (power-of-two) spacing of the FP32 anchor grid (mixed geometry) */
     double skin;        /* Verlet skin in use */
+    /* the anchor grid per axis: anchor A sits at anchor_lo + anchor_spacing * (A + anchor_origin);
+       anchor_bits is the packed width (anchor differences wrap by it on periodic axes) */
+    double anchor_lo[3], anchor_spacing[3], anchor_origin[3];
+    int anchor_bits[3];
 } orbc_observables;
 
 void orbc_config_default(orbc_config *cfg);
@@ -125,6 +130,16 @@ int orbc_set_state(orbc_ctx *ctx, int species, long long n, const double *x, con
                    const double *nrm, const double *o, const int *type, const unsigned *id);
 /* ProteContainer::bonds (src/container.h:30-34,138): {type, tag_i, tag_j} */
 int orbc_set_bonds(orbc_ctx *ctx, long long nb, const int *type, const unsigned *id_i, const unsigned *id_j);
+/* Ensemble members.  No reference counterpart: this is the batching of many
+ * independent systems (e.g. vesicles of uneven size) into one store and one
+ * launch per step.  Particle k of `species` (upload order) belongs to member
+ * group[k] in [0, n_groups); member g runs Langevin at kBT[g] (the noise
+ * amplitude of src/integrate_langevin.h:262-266 with that kBT).  Members must
+ * be placed apart in space (no pair between members is then in range); their
+ * noise streams are independent because Philox is keyed by particle id.
+ * Call after orbc_set_state of the species; n_groups is the same for both
+ * species.  orbc_run with the Nose-Hoover thermostat rejects members. */
+int orbc_set_groups(orbc_ctx *ctx, int species, long long n, const int *group, int n_groups, const double *kBT);
 int orbc_get_state(orbc_ctx *ctx, int species, double *x, double *v, double *nrm, double *o);
 /* Container::f and ::t (t = director force -dU/dn until a post-torque step
  * converts it).  e, w: per-particle energy and virial from the last

@@ -69,6 +69,8 @@ class Observables(C.Structure):
         ("kinetic", C.c_double), ("temperature", C.c_double), ("potential", C.c_double),
         ("virial", C.c_double), ("zeta", C.c_double), ("n", C.c_longlong), ("step", C.c_longlong),
         ("rebuilds", C.c_longlong), ("pairs", C.c_longlong), ("anchor_h", C.c_double), ("skin", C.c_double),
+        ("anchor_lo", C.c_double * 3), ("anchor_spacing", C.c_double * 3), ("anchor_origin", C.c_double * 3),
+        ("anchor_bits", C.c_int * 3),
     ]
 
 
@@ -99,6 +101,7 @@ SIGNATURES = [
     ("orbc_get_config", C.c_int, [_vp, C.POINTER(Config)]),
     ("orbc_set_state", C.c_int, [_vp, C.c_int, C.c_longlong, _d, _d, _d, _d, _i, _u]),
     ("orbc_set_bonds", C.c_int, [_vp, C.c_longlong, _i, _u, _u]),
+    ("orbc_set_groups", C.c_int, [_vp, C.c_int, C.c_longlong, _i, C.c_int, _d]),
     ("orbc_get_state", C.c_int, [_vp, C.c_int, _d, _d, _d, _d]),
     ("orbc_get_forces", C.c_int, [_vp, C.c_int, _d, _d, _d, _d]),
     ("orbc_set_forces", C.c_int, [_vp, C.c_int, _d, _d]),
@@ -246,6 +249,12 @@ class Simulation:
         b = np.ascontiguousarray(id_j, dtype=np.uint32)
         self._check(self.L.orbc_set_bonds(self.h, len(t), _ptr(t, _i), _ptr(a, _u), _ptr(b, _u)))
 
+    def set_groups(self, species: int, group, kBT):
+        """Ensemble members: group[k] of each particle of `species`, member temperatures kBT."""
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        k = np.ascontiguousarray(kBT, dtype=np.float64)
+        self._check(self.L.orbc_set_groups(self.h, int(species), len(g), _ptr(g, _i), len(k), _ptr(k, _d)))
+
     def get_state(self, species: int):
         m = self.n[species]
         out = {k: np.zeros((m, 3)) for k in ("x", "v", "n", "o")}
@@ -299,7 +308,11 @@ class Simulation:
     def observables(self) -> dict:
         o = Observables()
         self._check(self.L.orbc_get_observables(self.h, C.byref(o)))
-        return {k: getattr(o, k) for k, _ in Observables._fields_}
+        out = {}
+        for k, _ in Observables._fields_:
+            v = getattr(o, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
 
     def pairs(self) -> np.ndarray:
         """Interacting unordered pairs (id_i < id_j) of the current list, sorted."""
@@ -394,10 +407,10 @@ def compute_temperature(sim: Simulation) -> float:
 
 
 from .systems import (evans_fung, icosphere, init_rbc, load_example_large, planar_patch,  # noqa: E402
-                      random_sphere, system_to_simulation)
+                      random_sphere, system_to_simulation, vesicle_ensemble)
 
 __all__ = [
     "Config", "Container", "Simulation", "OrbcError", "default_config", "load_library",
     "compute_pairwise_fused", "compute_bonded", "integrate", "constrain_volume", "compute_temperature",
-    "evans_fung", "icosphere", "init_rbc", "load_example_large", "planar_patch", "random_sphere", "system_to_simulation",
+    "evans_fung", "icosphere", "init_rbc", "load_example_large", "planar_patch", "random_sphere", "system_to_simulation", "vesicle_ensemble",
 ]

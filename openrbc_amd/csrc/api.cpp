@@ -91,6 +91,7 @@ IntegArgs integ_args(orbc_ctx *c) {
     a.anc_grid = c->ag;
     a.dt = c->cfg.dt;
     langevin_coeffs(c->cfg, a.gamma, a.sigma);
+    a.grp = c->grp; a.sig_group = c->sig_group;
     a.seed = c->cfg.seed;
     a.step = (unsigned long long)c->step;
     for (int d = 0; d < 3; ++d) { a.box_lo[d] = c->cfg.box_lo[d]; a.box_hi[d] = c->cfg.box_hi[d]; }
@@ -203,6 +204,39 @@ void upload_bonds(orbc_ctx *c) {
     sync(c);
 }
 
+// ensemble members: per-particle member in device order and the noise
+// amplitude of every (member, type) from the member's kBT
+void upload_groups(orbc_ctx *c) {
+    if (c->group_kbt.empty()) return;
+    const long long n = c->n;
+    if (!c->grp) {
+        c->grp = c->mem.alloc<uint16_t>(n);
+        c->grp_alt = c->mem.alloc<uint16_t>(n);
+        c->sig_group = c->mem.alloc<double>(c->group_kbt.size() * NT + NT);
+    }
+    std::vector<uint16_t> img(n, 0);
+    long long k = 0;
+    for (int s = 0; s < 2; ++s) {
+        const Species &S = c->sp[s];
+        for (long long u = 0; u < S.n; ++u, ++k) img[k] = S.group.empty() ? 0 : S.group[u];
+    }
+    const size_t ng = c->group_kbt.size();
+    std::vector<double> sig(ng * NT);
+    for (size_t g = 0; g < ng; ++g) {
+        orbc_config cg = c->cfg;
+        cg.kBT = c->group_kbt[g];
+        double gam[NT], sg[NT];
+        langevin_coeffs(cg, gam, sg);
+        for (int t = 0; t < NT; ++t) sig[g * NT + t] = sg[t];
+    }
+    // the staging buffer holds 3n doubles: ample for n u16 in upload order
+    uint16_t *tmp = reinterpret_cast<uint16_t *>(c->stage);
+    ORBC_HIP_CHECK(hipMemcpyAsync(tmp, img.data(), 2 * n, hipMemcpyHostToDevice, c->stream));
+    launch_gather_u16(n, c->slot, tmp, c->grp, c->stream);
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->sig_group, sig.data(), 8 * sig.size(), hipMemcpyHostToDevice, c->stream));
+    sync(c);
+}
+
 void commit(orbc_ctx *c) {
     if (c->live) {
         voronoi_free(c);
@@ -252,6 +286,9 @@ void commit(orbc_ctx *c) {
     c->live = true;
     c->nh_started = false;
     c->nbonds = 0; c->kbmax = 0;
+    c->grp = c->grp_alt = nullptr;
+    c->sig_group = nullptr;
+    upload_groups(c);
     if (!c->b_type.empty()) upload_bonds(c);
     c->bonds_dirty = false;
     c->need_rebuild = true;
@@ -279,12 +316,27 @@ void set_cutoffs(orbc_ctx *c, double cs) {
         }
         rings[a] = std::max(1, (int)std::ceil(rmax / cs - 1e-12));
     }
+    c->far_rings = false;
+    for (int a = 0; a < NT; ++a) {
+        c->rings[a] = rings[a];
+        if ((c->types_present >> a & 1u) && rings[a] >= 2) c->far_rings = true;
+    }
     set_list_cutoffs(cut2, rings, c->stream);
 }
 
 void rebuild(orbc_ctx *c) {
     const long long n = c->n;
     hipStream_t s = c->stream;
+    int per[3];
+    double Lbox[3];
+    bool any_per = false;
+    for (int d = 0; d < 3; ++d) {
+        per[d] = c->cfg.periodic[d] != 0;
+        Lbox[d] = c->cfg.box_hi[d] - c->cfg.box_lo[d];
+        any_per = any_per || per[d];
+    }
+    // 0. periodic axes: positions into [lo, lo + L)
+    if (any_per) launch_wrap(c->x[c->cur], n, c->cfg.box_lo, Lbox, per, s);
     // 1. bounding box
     launch_bbox(c->x[c->cur], n, c->bbox, s);
     double bb[6];
@@ -297,8 +349,17 @@ void rebuild(orbc_ctx *c) {
     Grid g{};
     g.cs = cs;
     for (int d = 0; d < 3; ++d) {
-        g.lo[d] = bb[d] - 1e-6 * (1.0 + std::fabs(bb[d]));
-        g.dim[d] = std::max(1, (int)std::floor((bb[3 + d] - g.lo[d]) / cs) + 1);
+        if (per[d]) {
+            // cells tile the box exactly, so neighbour cells wrap
+            g.per[d] = 1;
+            g.lo[d] = c->cfg.box_lo[d];
+            g.dim[d] = std::max(1, (int)std::floor(Lbox[d] / cs));
+            g.csd[d] = Lbox[d] / (double)g.dim[d];
+        } else {
+            g.lo[d] = bb[d] - 1e-6 * (1.0 + std::fabs(bb[d]));
+            g.dim[d] = std::max(1, (int)std::floor((bb[3 + d] - g.lo[d]) / cs) + 1);
+            g.csd[d] = cs;
+        }
         g.bits[d] = ceil_log2(g.dim[d]);
     }
     if (g.bits[0] + g.bits[1] + g.bits[2] > 32) throw std::runtime_error("system extent exceeds the 32-bit Morton grid");
@@ -311,29 +372,53 @@ void rebuild(orbc_ctx *c) {
     }
     c->grid = g;
     set_cutoffs(c, cs);
-    // 3. anchor grid: smallest power-of-two spacing >= 1/4 whose anchors pack in 32 bits
+    if (any_per) {
+        int rmax = 1;
+        for (int a = 0; a < NT; ++a)
+            if (c->types_present >> a & 1u) rmax = std::max(rmax, c->rings[a]);
+        for (int d = 0; d < 3; ++d)
+            if (per[d] && g.dim[d] < 2 * rmax + 1)
+                throw std::runtime_error("periodic box too short for the cutoff: need at least 2 rings of cells");
+    }
+    // 3. anchor grid: the smallest power-of-two spacing h >= 1/4 whose anchors
+    // pack in 32 bits.  Open axes use h itself; a periodic axis uses
+    // 2^bits >= L/h anchors of spacing L / 2^bits, so differences wrap by bits.
     Anchor ag{};
-    for (int k = -2; k < 40; ++k) {
+    bool packed = false;
+    for (int k = -2; k < 40 && !packed; ++k) {
         const double h = std::ldexp(1.0, k);
         int bits[3];
         long long A0[3];
         int tot = 0;
         for (int d = 0; d < 3; ++d) {
-            A0[d] = (long long)std::floor(bb[d] / h) - 1;
-            const long long span = (long long)std::floor(bb[3 + d] / h) - A0[d] + 2;
-            bits[d] = std::max(1, ceil_log2(span));
+            if (per[d]) {
+                A0[d] = 0;
+                bits[d] = std::max(1, ceil_log2((long long)std::ceil(Lbox[d] / h)));
+            } else {
+                A0[d] = (long long)std::floor(bb[d] / h) - 1;
+                const long long span = (long long)std::floor(bb[3 + d] / h) - A0[d] + 2;
+                bits[d] = std::max(1, ceil_log2(span));
+            }
             tot += bits[d];
         }
         if (tot <= 32) {
-            ag.hd = h; ag.h = (float)h;
-            for (int d = 0; d < 3; ++d) ag.A0[d] = A0[d];
+            for (int d = 0; d < 3; ++d) {
+                ag.hd[d] = per[d] ? Lbox[d] / std::ldexp(1.0, bits[d]) : h;
+                ag.hf[d] = (float)ag.hd[d];
+                ag.lo[d] = per[d] ? c->cfg.box_lo[d] : 0.0;
+                ag.A0[d] = A0[d];
+                ag.wrap[d] = per[d] ? (unsigned)(32 - bits[d]) : 0u;
+                ag.L[d] = per[d] ? Lbox[d] : 0.0;
+                ag.Linv[d] = per[d] ? 1.0 / Lbox[d] : 0.0;
+                c->anchor_bits[d] = bits[d];
+            }
             ag.shy = bits[0]; ag.shz = bits[0] + bits[1];
             ag.mx = (1u << bits[0]) - 1u; ag.my = (1u << bits[1]) - 1u;
             ag.mz = bits[2] >= 32 ? 0xffffffffu : ((1u << bits[2]) - 1u);
-            break;
+            packed = true;
         }
     }
-    if (ag.hd == 0) throw std::runtime_error("cannot pack anchors");
+    if (!packed) throw std::runtime_error("cannot pack anchors");
     c->ag = ag;
     // 4. Morton sort of the particles by grid cell
     launch_keys(c->x[c->cur], n, g, c->keys, c->vals, s);
@@ -363,6 +448,7 @@ void rebuild(orbc_ctx *c) {
     p.id_src = c->id; p.id_dst = c->id_alt;
     p.slot_src = c->slot; p.slot_dst = c->slot_alt;
     p.aff_src = c->voro.aff; p.aff_dst = c->voro.aff_alt;
+    p.grp_src = c->grp; p.grp_dst = c->grp_alt;
     launch_permute(p, s);
     c->cur = nx;
     std::swap(c->v, c->v_alt); std::swap(c->o, c->o_alt);
@@ -370,6 +456,7 @@ void rebuild(orbc_ctx *c) {
     std::swap(c->f64, c->f64_alt); std::swap(c->t64, c->t64_alt);
     std::swap(c->type, c->type_alt); std::swap(c->id, c->id_alt); std::swap(c->slot, c->slot_alt);
     if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
+    if (c->grp) std::swap(c->grp, c->grp_alt);
     launch_inverse(c->slot, c->inv, n, s);
     // 6. mirrors, anchors, build-time offsets
     launch_refresh(n, c->x[c->cur], c->nrm[c->cur], c->type, ag, c->pq[c->cur], c->rb, c->anc, s);
@@ -393,7 +480,7 @@ void rebuild(orbc_ctx *c) {
             a.aff = c->voro.aff; a.centroids = c->voro.c;
         }
         zero_flag(c, 2);
-        launch_nlist(a, c->keys, nocc, s);
+        launch_nlist(a, c->keys, nocc, c->far_rings, s);
         const unsigned mc = read_flag(c, 2);
         if ((int)mc <= c->kmax) break;
         c->mem.release(c->nbr);
@@ -476,6 +563,8 @@ int orbc_create(const orbc_config *cfg, orbc_ctx **out) {
     const orbc_config &cfg_ = cfg ? *cfg : c0;
     if (cfg_.dt <= 0 || cfg_.skin <= 0 || cfg_.kBT < 0 || cfg_.eta < 0) return ORBC_ERR_INVALID;
     if (cfg_.precision != ORBC_FP64 && cfg_.precision != ORBC_MIXED) return ORBC_ERR_INVALID;
+    for (int d = 0; d < 3; ++d)
+        if (cfg_.periodic[d] && !(cfg_.box_hi[d] > cfg_.box_lo[d])) return ORBC_ERR_INVALID;
     // src/runtime_parameter.h:118-120 (reported, not exit(0))
     if (cfg_.freq_voronoi <= 0 || cfg_.freq_sort_ctrd % cfg_.freq_voronoi || cfg_.freq_sort_bond % cfg_.freq_voronoi ||
         cfg_.freq_cleanup % cfg_.freq_voronoi)
@@ -563,6 +652,28 @@ int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const
             voronoi_free(c);
             c->mem.free_all();
         }
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_set_groups(orbc_ctx *c, int species, long long n, const int *group, int n_groups, const double *kBT) {
+    return guarded(c, [&] {
+        if (species != ORBC_LIPID && species != ORBC_PROTEIN) return fail(c, ORBC_ERR_INVALID, "bad species");
+        Species &S = c->sp[species];
+        if (n != S.n) return fail(c, ORBC_ERR_INVALID, "group array length differs from the species' particle count");
+        if (n_groups < 1 || n_groups > 65535 || !kBT) return fail(c, ORBC_ERR_INVALID, "n_groups must be 1..65535");
+        for (int g = 0; g < n_groups; ++g)
+            if (!(kBT[g] >= 0)) return fail(c, ORBC_ERR_INVALID, "negative member temperature");
+        if (!c->group_kbt.empty() && (int)c->group_kbt.size() != n_groups)
+            return fail(c, ORBC_ERR_INVALID, "n_groups differs from the earlier call");
+        std::vector<uint16_t> gv((size_t)n);
+        for (long long k = 0; k < n; ++k) {
+            if (!group || group[k] < 0 || group[k] >= n_groups) return fail(c, ORBC_ERR_INVALID, "group out of range");
+            gv[k] = (uint16_t)group[k];
+        }
+        S.group.swap(gv);
+        c->group_kbt.assign(kBT, kBT + n_groups);
+        if (c->live) upload_groups(c);
         return (int)ORBC_OK;
     });
 }
@@ -747,6 +858,7 @@ static StepArgs step_args(orbc_ctx *c) {
     a.anc_grid = c->ag;
     a.dt = c->cfg.dt;
     langevin_coeffs(c->cfg, a.gamma, a.sigma);
+    a.grp = c->grp; a.sig_group = c->sig_group;
     a.seed = c->cfg.seed;
     a.step = (unsigned long long)c->step;
     a.dslot = c->flags + 4;
@@ -888,6 +1000,8 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 return fail(c, ORBC_ERR_NAN, e.what());
             }
         } else {
+            if (!c->group_kbt.empty())
+                return fail(c, ORBC_ERR_INVALID, "ensemble members with their own kBT need the Langevin thermostat");
             // Nose-Hoover, src/openrbc.cpp:176-243 with FUSED_INTEGRATOR
             IntegArgs a;
             const double Q = nh_Q(c), nkT = 1.5 * (double)c->n * c->cfg.kBT;
@@ -988,7 +1102,15 @@ int orbc_get_observables(orbc_ctx *c, orbc_observables *obs) {
         long long tot = 0;
         for (auto v : cnt) tot += v;
         obs->pairs = tot;
-        obs->anchor_h = c->ag.hd;
+        obs->anchor_h = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            if (!c->cfg.periodic[d]) obs->anchor_h = c->ag.hd[d];
+            obs->anchor_lo[d] = c->ag.lo[d];
+            obs->anchor_spacing[d] = c->ag.hd[d];
+            obs->anchor_origin[d] = (double)c->ag.A0[d];
+            obs->anchor_bits[d] = c->cfg.periodic[d] ? c->anchor_bits[d] : 0;
+        }
+        if (obs->anchor_h == 0.0) obs->anchor_h = c->ag.hd[0];
         obs->skin = c->cfg.skin;
         return (int)ORBC_OK;
     });

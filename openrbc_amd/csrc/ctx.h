@@ -46,6 +46,7 @@ struct Species {
     std::vector<double> x, v, nrm, o;
     std::vector<int> type;
     std::vector<uint32_t> id;
+    std::vector<uint16_t> group;     // ensemble member per particle (empty: member 0)
 };
 
 struct orbc_ctx {
@@ -135,6 +136,13 @@ struct orbc_ctx {
         Gate gate;                         // skip gate of the batched run (VoroArgs::gate)
     } voro;
     double list_life = 0.0;       // steps a neighbour list lasts on average (batch sizing)
+    // ensemble members (orbc_set_groups): per-particle member and per-member kBT
+    std::vector<double> group_kbt;
+    uint16_t *grp = nullptr, *grp_alt = nullptr;
+    double *sig_group = nullptr;
+    bool far_rings = false;       // a present type searches past the first ring of cells
+    int rings[6] = {1, 1, 1, 1, 1, 1};   // grid rings each type searches
+    int anchor_bits[3] = {0, 0, 0};      // anchor bits per axis (wrap width on periodic axes)
     // volume constraint inside orbc_run (cfg.volume_target != 0)
     double vol_target = 0.0;
     bool vol_target_set = false;

@@ -54,6 +54,20 @@ __global__ __launch_bounds__(BLOCK) void k_bbox_partial(const double *x, long lo
     }
 }
 
+struct WrapArgs {
+    double lo[3], L[3];
+    int per[3];
+};
+__global__ __launch_bounds__(BLOCK) void k_wrap(double *x, long long n, WrapArgs w) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    for (int d = 0; d < 3; ++d)
+        if (w.per[d]) {
+            const double u = x[3 * i + d] - w.lo[d];
+            x[3 * i + d] = w.lo[d] + (u - w.L[d] * floor(u / w.L[d]));
+        }
+}
+
 // one wavefront: lanes stride over the block partials, then a shuffle reduction
 __global__ void k_bbox_final(const double *partial, int nb, double *out) {
     double r[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
@@ -92,9 +106,9 @@ __global__ __launch_bounds__(BLOCK) void k_keys(const double *x, long long n, Gr
                                                 uint32_t *vals) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const int cx = cell_coord(x[3 * i], g.lo[0], g.cs, g.dim[0]);
-    const int cy = cell_coord(x[3 * i + 1], g.lo[1], g.cs, g.dim[1]);
-    const int cz = cell_coord(x[3 * i + 2], g.lo[2], g.cs, g.dim[2]);
+    const int cx = cell_coord(x[3 * i], g.lo[0], g.csd[0], g.dim[0]);
+    const int cy = cell_coord(x[3 * i + 1], g.lo[1], g.csd[1], g.dim[1]);
+    const int cz = cell_coord(x[3 * i + 2], g.lo[2], g.csd[2], g.dim[2]);
     keys[i] = morton_var(cx, cy, cz, g.bits[0], g.bits[1], g.bits[2]);
     vals[i] = (uint32_t)i;
 }
@@ -121,6 +135,7 @@ __global__ __launch_bounds__(BLOCK) void k_permute(PermuteArgs a) {
     a.id_dst[i] = a.id_src[p];
     a.slot_dst[i] = a.slot_src[p];
     if (a.aff_src) a.aff_dst[i] = a.aff_src[p];
+    if (a.grp_src) a.grp_dst[i] = a.grp_src[p];
 }
 
 __global__ __launch_bounds__(BLOCK) void k_inverse(const uint32_t *slot, uint32_t *inv, long long n) {
@@ -138,11 +153,16 @@ __global__ __launch_bounds__(BLOCK) void k_refresh(long long n, const double *x,
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double v = x[3 * i + d];
-        A[d] = (long long)floor(v / a.hd);
-        r[d] = __double2float_rn(v - a.hd * (double)A[d]);
+        long long q = (long long)floor((v - a.lo[d]) / a.hd[d]) - a.A0[d];
+        if (a.wrap[d]) {
+            // periodic axis: x is wrapped into the box; guard the last anchor against rounding
+            const long long M = 1LL << (32 - a.wrap[d]);
+            q = q < 0 ? 0 : (q > M - 1 ? M - 1 : q);
+        }
+        A[d] = q;
+        r[d] = __double2float_rn(v - anchor_pos(a, d, q));
     }
-    const uint32_t w = (uint32_t)(A[0] - a.A0[0]) | ((uint32_t)(A[1] - a.A0[1]) << a.shy) |
-                       ((uint32_t)(A[2] - a.A0[2]) << a.shz);
+    const uint32_t w = (uint32_t)A[0] | ((uint32_t)A[1] << a.shy) | ((uint32_t)A[2] << a.shz);
     anc[i] = w;
     rb[3 * i] = r[0];
     rb[3 * i + 1] = r[1];
@@ -160,13 +180,27 @@ __device__ inline long long cell_lin(int cx, int cy, int cz, const Grid &g) {
     return ((long long)cz * g.dim[1] + cy) * g.dim[0] + cx;
 }
 
+// cell coordinate c + dc along axis d: wraps on a periodic axis; false if it
+// leaves an open axis
+__device__ inline bool cell_step(const Grid &g, int d, int c, int dc, int &out) {
+    int v = c + dc;
+    if (g.per[d]) {
+        v %= g.dim[d];
+        if (v < 0) v += g.dim[d];
+    } else if (v < 0 || v >= g.dim[d]) {
+        return false;
+    }
+    out = v;
+    return true;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_cell_ranges(const double *x, long long n, Grid g, uint32_t *cstart,
                                                        uint32_t *cend) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     auto lin = [&](long long k) {
-        return cell_lin(cell_coord(x[3 * k], g.lo[0], g.cs, g.dim[0]), cell_coord(x[3 * k + 1], g.lo[1], g.cs, g.dim[1]),
-                        cell_coord(x[3 * k + 2], g.lo[2], g.cs, g.dim[2]), g);
+        return cell_lin(cell_coord(x[3 * k], g.lo[0], g.csd[0], g.dim[0]), cell_coord(x[3 * k + 1], g.lo[1], g.csd[1], g.dim[1]),
+                        cell_coord(x[3 * k + 2], g.lo[2], g.csd[2], g.dim[2]), g);
     };
     const long long c = lin(i);
     const long long cp = i > 0 ? lin(i - 1) : -1;
@@ -194,9 +228,9 @@ __device__ inline bool nl_accept(const NlCtx &c, uint32_t j, float4 xj, int tj, 
     if (!(cut2 > 0.f)) return false;
     const Anchor &g = c.a.anc;
     const uint32_t w = __float_as_uint(xj.w);
-    const float dx = (c.xi.x - xj.x) + g.h * (float)(c.ax - (int)(w & g.mx));
-    const float dy = (c.xi.y - xj.y) + g.h * (float)(c.ay - (int)((w >> g.shy) & g.my));
-    const float dz = (c.xi.z - xj.z) + g.h * (float)(c.az - (int)(w >> g.shz));
+    const float dx = (c.xi.x - xj.x) + anchor_dx(g, 0, c.ax - (int)(w & g.mx));
+    const float dy = (c.xi.y - xj.y) + anchor_dx(g, 1, c.ay - (int)((w >> g.shy) & g.my));
+    const float dz = (c.xi.z - xj.z) + anchor_dx(g, 2, c.az - (int)(w >> g.shz));
     if (!(dx * dx + dy * dy + dz * dz < cut2)) return false;
     if (c.a.aff) {
         // reference stencil radii: pp 9, pl 8, ll 6 (src/compute_pairwise_fused.h:141-284)
@@ -240,9 +274,9 @@ __device__ inline void nl_wave_shell(NlCtx &c, int cx, int cy, int cz, int R, fl
         uint32_t st = 0, cntc = 0;
         if (k < ncube) {
             const int dx = k % D - R, dy = (k / D) % D - R, dz = k / (D * D) - R;
-            const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
-            if (max(abs(dx), max(abs(dy), abs(dz))) >= 2 && xx >= 0 && yy >= 0 && zz >= 0 && xx < g.dim[0] &&
-                yy < g.dim[1] && zz < g.dim[2]) {
+            int xx, yy, zz;
+            if (max(abs(dx), max(abs(dy), abs(dz))) >= 2 && cell_step(g, 0, cx, dx, xx) && cell_step(g, 1, cy, dy, yy) &&
+                cell_step(g, 2, cz, dz, zz)) {
                 const long long cl = cell_lin(xx, yy, zz, g);
                 st = c.a.cstart[cl];
                 cntc = c.a.cend[cl] - st;
@@ -299,8 +333,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_flags(const double *x, long long
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     auto lin = [&](long long k) {
-        return cell_lin(cell_coord(x[3 * k], g.lo[0], g.cs, g.dim[0]), cell_coord(x[3 * k + 1], g.lo[1], g.cs, g.dim[1]),
-                        cell_coord(x[3 * k + 2], g.lo[2], g.cs, g.dim[2]), g);
+        return cell_lin(cell_coord(x[3 * k], g.lo[0], g.csd[0], g.dim[0]), cell_coord(x[3 * k + 1], g.lo[1], g.csd[1], g.dim[1]),
+                        cell_coord(x[3 * k + 2], g.lo[2], g.csd[2], g.dim[2]), g);
     };
     flag[i] = (i == 0 || lin(i) != lin(i - 1)) ? 1u : 0u;
 }
@@ -316,16 +350,16 @@ __global__ __launch_bounds__(BLOCK) void k_cell_compact(const uint32_t *flag, co
 // stripe) pairs -- P = cell particles rounded up to a power of two, S = 64/P
 // stripes -- and each round tests S consecutive candidates for every
 // particle.  Hits are appended in candidate order with a ballot-based
-// compaction, so the list order is deterministic.  Particle types whose
-// cutoff + skin exceeds one cell (band-3 / actin with the sigma-3.4 LJ) scan
-// the second ring of cells themselves afterwards.
+// compaction, so the list order is deterministic.  The test is branch-free:
+// a candidate's index/type word and its position are read from LDS together,
+// and the list cutoff comes from the particle type's row held in registers.
+// Particle types whose cutoff + skin exceeds one cell (band-3 / actin with the
+// sigma-3.4 LJ) get the outer shells appended by k_nlist_far afterwards.
 __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *occ, long long nocc) {
     __shared__ float4 s_x[BLOCK / 64][NL_CAP];
     __shared__ uint32_t s_j[BLOCK / 64][NL_CAP];
     __shared__ uint32_t s_pref[BLOCK / 64][28];
     __shared__ uint32_t s_cst[BLOCK / 64][27];
-    __shared__ uint32_t s_pref2[BLOCK / 64][65];
-    __shared__ uint32_t s_cst2[BLOCK / 64][64];
     __shared__ float s_cut2[36];
     if (threadIdx.x < 36) s_cut2[threadIdx.x] = c_list_cut2[threadIdx.x];
     __syncthreads();
@@ -338,14 +372,15 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
     if (cell >= nocc) return;                       // whole wave; no block barrier below
     const Grid &g = a.g;
     const long long p0 = occ[cell], p1 = (cell + 1 < nocc) ? occ[cell + 1] : a.n;
-    const int cx = cell_coord(a.x[3 * p0], g.lo[0], g.cs, g.dim[0]);
-    const int cy = cell_coord(a.x[3 * p0 + 1], g.lo[1], g.cs, g.dim[1]);
-    const int cz = cell_coord(a.x[3 * p0 + 2], g.lo[2], g.cs, g.dim[2]);
+    const int cx = cell_coord(a.x[3 * p0], g.lo[0], g.csd[0], g.dim[0]);
+    const int cy = cell_coord(a.x[3 * p0 + 1], g.lo[1], g.csd[1], g.dim[1]);
+    const int cz = cell_coord(a.x[3 * p0 + 2], g.lo[2], g.csd[2], g.dim[2]);
     // ranges of the 27 neighbour cells, prefix counts
     uint32_t st = 0, cntc = 0;
     if (lane < 27) {
-        const int xx = cx + lane % 3 - 1, yy = cy + (lane / 3) % 3 - 1, zz = cz + lane / 9 - 1;
-        if (xx >= 0 && yy >= 0 && zz >= 0 && xx < g.dim[0] && yy < g.dim[1] && zz < g.dim[2]) {
+        int xx, yy, zz;
+        if (cell_step(g, 0, cx, lane % 3 - 1, xx) && cell_step(g, 1, cy, (lane / 3) % 3 - 1, yy) &&
+            cell_step(g, 2, cz, lane / 9 - 1, zz)) {
             const long long cl = cell_lin(xx, yy, zz, g);
             st = a.cstart[cl];
             cntc = a.cend[cl] - st;
@@ -361,6 +396,7 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
     if (lane == 0) s_pref[w][0] = 0;
     wave_sync();
     const long long np = p1 - p0;
+    const Anchor &an = a.anc;
     for (long long pc = 0; pc < np; pc += 64) {
         const int npc = (int)min(64LL, np - pc);
         int P = 1;
@@ -370,11 +406,16 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
         const bool act = pslot < npc;
         const long long i = p0 + pc + (act ? pslot : 0);
         NlCtx c{a, i, 0, make_float4(0, 0, 0, 0), 0, 0, 0, 0.f, 0.f, 0.f, 0};
+        float crow[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) crow[t] = -1.f;
         if (act) {
             c.ti = a.type[i];
             c.xi = a.pq[i].x;
             const uint32_t wi = __float_as_uint(c.xi.w);
-            c.ax = (int)(wi & a.anc.mx); c.ay = (int)((wi >> a.anc.shy) & a.anc.my); c.az = (int)(wi >> a.anc.shz);
+            c.ax = (int)(wi & an.mx); c.ay = (int)((wi >> an.shy) & an.my); c.az = (int)(wi >> an.shz);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) crow[t] = s_cut2[c.ti * NT + t];
             if (a.aff) {
                 const int ca = a.aff[i];
                 c.c0 = a.centroids[3 * ca]; c.c1 = a.centroids[3 * ca + 1]; c.c2 = a.centroids[3 * ca + 2];
@@ -382,6 +423,7 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
         }
         const unsigned long long gmask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << (pslot * S));
         const unsigned long long below = (stripe == 0) ? 0ull : (((1ull << stripe) - 1ull) << (pslot * S));
+        uint32_t *const row = a.nbr + i;
         int cnt = 0;
         for (uint32_t off = 0; off < total; off += NL_CAP) {
             const uint32_t m = min((uint32_t)NL_CAP, total - off);
@@ -400,37 +442,82 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
             wave_sync();
             for (uint32_t t0 = 0; t0 < m; t0 += S) {
                 const uint32_t t = t0 + stripe;
-                bool hit = false;
-                uint32_t j = 0;
-                if (act && t < m) {
-                    const uint32_t jt = s_j[w][t];
-                    j = jt & 0x1fffffffu;
-                    hit = nl_accept(c, j, s_x[w][t], (int)(jt >> 29), s_cut2);
+                const bool valid = act && t < m;
+                const uint32_t tt = valid ? t : 0u;
+                const uint32_t jt = s_j[w][tt];
+                const float4 xj = s_x[w][tt];
+                const uint32_t j = jt & 0x1fffffffu;
+                const int tj = (int)(jt >> 29);
+                float cut2 = crow[0];
+#pragma unroll
+                for (int q2 = 1; q2 < NT; ++q2) cut2 = (tj == q2) ? crow[q2] : cut2;
+                const uint32_t wj = __float_as_uint(xj.w);
+                const float dx = (c.xi.x - xj.x) + anchor_dx(an, 0, c.ax - (int)(wj & an.mx));
+                const float dy = (c.xi.y - xj.y) + anchor_dx(an, 1, c.ay - (int)((wj >> an.shy) & an.my));
+                const float dz = (c.xi.z - xj.z) + anchor_dx(an, 2, c.az - (int)(wj >> an.shz));
+                // a pair without interaction has cut2 < 0 and never passes
+                bool hit = valid && j != (uint32_t)i && dx * dx + dy * dy + dz * dz < cut2;
+                if (a.aff && hit) {
+                    // reference stencil radii: pp 9, pl 8, ll 6 (src/compute_pairwise_fused.h:141-284)
+                    const int cb = a.aff[j];
+                    const float d2 = normsq_unfused(__fsub_rn(a.centroids[3 * cb], c.c0),
+                                                    __fsub_rn(a.centroids[3 * cb + 1], c.c1),
+                                                    __fsub_rn(a.centroids[3 * cb + 2], c.c2));
+                    const int nl = (c.ti == 0) + (tj == 0);
+                    const float Rs = nl == 2 ? 6.0f : (nl == 1 ? 8.0f : 9.0f);
+                    hit = d2 < Rs * Rs;
                 }
                 const unsigned long long hm = __ballot(hit);
                 if (hit) {
                     const int pos = cnt + __popcll(hm & below);
-                    if (pos < a.kmax) a.nbr[(long long)pos * a.stride + i] = j;
+                    if (pos < a.kmax) row[(size_t)pos * (size_t)a.stride] = j;
                 }
                 cnt += __popcll(hm & gmask);
             }
         }
-        // particles whose type reaches past the first ring: one at a time,
-        // the whole wave scans the outer shells for it
-        unsigned long long far = __ballot(act && stripe == 0 && c_rings[c.ti] >= 2);
-        while (far) {
-            const int src = __ffsll((long long)far) - 1;
-            far &= far - 1;
-            NlCtx cb{a, (long long)__shfl((int)i, src, 64), __shfl(c.ti, src, 64),
-                     make_float4(__shfl(c.xi.x, src, 64), __shfl(c.xi.y, src, 64), __shfl(c.xi.z, src, 64), 0.f),
-                     __shfl(c.ax, src, 64), __shfl(c.ay, src, 64), __shfl(c.az, src, 64),
-                     __shfl(c.c0, src, 64), __shfl(c.c1, src, 64), __shfl(c.c2, src, 64), __shfl(cnt, src, 64)};
-            nl_wave_shell(cb, cx, cy, cz, c_rings[cb.ti], s_x[w], s_j[w], s_pref2[w], s_cst2[w], s_cut2);
-            if (lane / S == src / S) cnt = cb.cnt;
-        }
         if (act && stripe == 0) a.nnbr[i] = (uint32_t)(cnt < a.kmax ? cnt : a.kmax);
         const int mc = wave_max_i((act && stripe == 0) ? cnt : 0);
         if (lane == 0 && mc > a.kmax) atomicMax(a.max_count, (unsigned)mc);
+    }
+}
+
+// Outer shells (Chebyshev distance 2..R) for the particles whose type reaches
+// past the first ring of cells, appended after the k_nlist entries.  One
+// wavefront per 64 particles; each such particle is handled by the whole wave
+// (nl_wave_shell).  nnbr holds the first-ring count on entry.
+__global__ __launch_bounds__(BLOCK) void k_nlist_far(NlistArgs a) {
+    __shared__ float4 s_x[BLOCK / 64][NL_CAP];
+    __shared__ uint32_t s_j[BLOCK / 64][NL_CAP];
+    __shared__ uint32_t s_pref2[BLOCK / 64][65];
+    __shared__ uint32_t s_cst2[BLOCK / 64][64];
+    __shared__ float s_cut2[36];
+    if (threadIdx.x < 36) s_cut2[threadIdx.x] = c_list_cut2[threadIdx.x];
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = __lane_id();
+    const long long base = ((long long)blockIdx.x * (BLOCK / 64) + w) * 64;
+    const long long il = base + lane;
+    unsigned long long far = __ballot(il < a.n && c_rings[a.type[il]] >= 2);
+    const Grid &g = a.g;
+    const Anchor &an = a.anc;
+    while (far) {
+        const int src = __ffsll((long long)far) - 1;
+        far &= far - 1;
+        const long long i = base + src;              // wave-uniform
+        NlCtx c{a, i, (int)a.type[i], a.pq[i].x, 0, 0, 0, 0.f, 0.f, 0.f, (int)a.nnbr[i]};
+        const uint32_t wi = __float_as_uint(c.xi.w);
+        c.ax = (int)(wi & an.mx); c.ay = (int)((wi >> an.shy) & an.my); c.az = (int)(wi >> an.shz);
+        if (a.aff) {
+            const int ca = a.aff[i];
+            c.c0 = a.centroids[3 * ca]; c.c1 = a.centroids[3 * ca + 1]; c.c2 = a.centroids[3 * ca + 2];
+        }
+        const int cx = cell_coord(a.x[3 * i], g.lo[0], g.csd[0], g.dim[0]);
+        const int cy = cell_coord(a.x[3 * i + 1], g.lo[1], g.csd[1], g.dim[1]);
+        const int cz = cell_coord(a.x[3 * i + 2], g.lo[2], g.csd[2], g.dim[2]);
+        nl_wave_shell(c, cx, cy, cz, c_rings[c.ti], s_x[w], s_j[w], s_pref2[w], s_cst2[w], s_cut2);
+        if (lane == 0) {
+            a.nnbr[i] = (uint32_t)(c.cnt < a.kmax ? c.cnt : a.kmax);
+            if (c.cnt > a.kmax) atomicMax(a.max_count, (unsigned)c.cnt);
+        }
     }
 }
 
@@ -454,10 +541,9 @@ __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, co
     float d2 = 0.f;
     if (i < n) {
         const uint32_t w = anc[i];
-        const long long A[3] = {(long long)(w & g.mx) + g.A0[0], (long long)((w >> g.shy) & g.my) + g.A0[1],
-                                (long long)(w >> g.shz) + g.A0[2]};
+        const long long A[3] = {(long long)(w & g.mx), (long long)((w >> g.shy) & g.my), (long long)(w >> g.shz)};
         for (int d = 0; d < 3; ++d) {
-            const float e = __double2float_rn(x[3 * i + d] - g.hd * (double)A[d]) - rb[3 * i + d];
+            const float e = __double2float_rn(x[3 * i + d] - anchor_pos(g, d, A[d])) - rb[3 * i + d];
             d2 += e * e;
         }
         if (!isfinite(d2)) d2 = 3.0e38f;
@@ -483,6 +569,13 @@ void set_list_cutoffs(const float *cut2_list, const int *rings, hipStream_t s) {
                                           hipMemcpyHostToDevice, s));
     ORBC_HIP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rings), rings, sizeof(int) * NT, 0,
                                           hipMemcpyHostToDevice, s));
+}
+
+void launch_wrap(double *x, long long n, const double lo[3], const double L[3], const int per[3], hipStream_t s) {
+    if (n <= 0) return;
+    WrapArgs w{};
+    for (int d = 0; d < 3; ++d) { w.lo[d] = lo[d]; w.L[d] = L[d]; w.per[d] = per[d]; }
+    hipLaunchKernelGGL(k_wrap, dim3(nblk(n)), dim3(BLOCK), 0, s, x, n, w);
 }
 
 void launch_bbox(const double *x, long long n, double *out6, hipStream_t s) {
@@ -534,10 +627,11 @@ long long launch_cell_list(const double *x, long long n, Grid g, uint32_t *flag,
     return (long long)last[0] + last[1];
 }
 
-void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, hipStream_t s) {
+void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool far, hipStream_t s) {
     if (a.n <= 0 || nocc <= 0) return;
     const long long nb = (nocc + BLOCK / 64 - 1) / (BLOCK / 64);
     hipLaunchKernelGGL(k_nlist, dim3((unsigned)nb), dim3(BLOCK), 0, s, a, occ, nocc);
+    if (far) hipLaunchKernelGGL(k_nlist_far, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a);
 }
 
 void launch_bondlist(long long n, long long stride, const uint32_t *slot, const uint32_t *inv,

@@ -88,6 +88,41 @@ __device__ inline void radius(double r2, double &r, double &rinv) {
     r = r2 * rinv;
 }
 
+// Anisotropic polynomial term of one directed pair inside its cutoff
+// (src/pairwise_kernel.h:37-62): force on i, director force on i, energy and
+// virial halves.
+template <typename R, bool ENERGY>
+__device__ __forceinline__ void aniso_term(R r2, R dx, R dy, R dz, R cut, R rep, R att, R alpha, R nix, R niy, R niz,
+                                           R njx, R njy, R njz, Acc<R> &acc) {
+    R r, rinv;
+    radius(r2, r, rinv);
+    const R ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+    const R s = cut - r;
+    const R s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+    const R ninj = nix * njx + niy * njy + niz * njz;
+    const R nie = nix * ex + niy * ey + niz * ez;
+    const R nje = njx * ex + njy * ey + njz * ez;
+    const R a = ninj - nie * nje;
+    const R A = R(1) + alpha * (a - R(1));
+    const R pix = nix - nie * ex, piy = niy - nie * ey, piz = niz - nie * ez;
+    const R pjx = njx - nje * ex, pjy = njy - nje * ey, pjz = njz - nje * ez;
+    const R ua = att * s4;
+    const R aua = alpha * ua;
+    const R auar = aua * rinv;
+    const R fra = s3 * (R(8) * rep * s4 + R(4) * A * att);
+    const R fx = fra * ex + auar * (nje * pix + nie * pjx);
+    const R fy = fra * ey + auar * (nje * piy + nie * pjy);
+    const R fz = fra * ez + auar * (nje * piz + nie * pjz);
+    acc.tx -= aua * pjx;
+    acc.ty -= aua * pjy;
+    acc.tz -= aua * pjz;
+    acc.fx += fx; acc.fy += fy; acc.fz += fz;
+    if (ENERGY) {
+        acc.e += R(0.5) * s4 * (rep * s4 + A * att);
+        acc.w += R(0.5) * (dx * fx + dy * fy + dz * fz);
+    }
+}
+
 // One directed pair (owner i, neighbour j, d = x_i - x_j).  Returns true if
 // the pair interacts (same selection test as the oracle, bit for bit).
 template <typename R, bool ENERGY>
@@ -97,39 +132,21 @@ __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R
     const int k = ti * NT + tj;
     R fx, fy, fz;
     if (r2 < t.cut2p[k]) {
+        if (t.aniso[k]) {
+            aniso_term<R, ENERGY>(r2, dx, dy, dz, t.cutp[k], t.rep[k], t.att[k], t.alpha[k], nix, niy, niz, njx, njy,
+                                  njz, acc);
+            return true;
+        }
+        // src/pairwise_kernel_fused.h:79-97
         R r, rinv;
         radius(r2, r, rinv);
         const R ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
         const R s = t.cutp[k] - r;
-        const R s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
-        const R rep = t.rep[k], att = t.att[k];
-        if (t.aniso[k]) {
-            // src/pairwise_kernel.h:37-62
-            const R alpha = t.alpha[k];
-            const R ninj = nix * njx + niy * njy + niz * njz;
-            const R nie = nix * ex + niy * ey + niz * ez;
-            const R nje = njx * ex + njy * ey + njz * ez;
-            const R a = ninj - nie * nje;
-            const R A = R(1) + alpha * (a - R(1));
-            const R pix = nix - nie * ex, piy = niy - nie * ey, piz = niz - nie * ez;
-            const R pjx = njx - nje * ex, pjy = njy - nje * ey, pjz = njz - nje * ez;
-            const R ua = att * s4;
-            const R aua = alpha * ua;
-            const R auar = aua * rinv;
-            const R fra = s3 * (R(8) * rep * s4 + R(4) * A * att);
-            fx = fra * ex + auar * (nje * pix + nie * pjx);
-            fy = fra * ey + auar * (nje * piy + nie * pjy);
-            fz = fra * ez + auar * (nje * piz + nie * pjz);
-            acc.tx -= aua * pjx;
-            acc.ty -= aua * pjy;
-            acc.tz -= aua * pjz;
-            if (ENERGY) acc.e += R(0.5) * s4 * (rep * s4 + A * att);
-        } else {
-            // src/pairwise_kernel_fused.h:79-97
-            const R fr = R(8) * rep * s3 * s4;
-            fx = fr * ex; fy = fr * ey; fz = fr * ez;
-            if (ENERGY) acc.e += R(0.5) * rep * s4 * s4;
-        }
+        const R s3 = s * s * s, s4 = (s * s) * (s * s);
+        const R rep = t.rep[k];
+        const R fr = R(8) * rep * s3 * s4;
+        fx = fr * ex; fy = fr * ey; fz = fr * ez;
+        if (ENERGY) acc.e += R(0.5) * rep * s4 * s4;
     } else {
         if (!(r2 < t.cut2lj[k])) return false;
         // src/pairwise_kernel_fused.h:63-77 (WCA, unshifted)
@@ -144,7 +161,6 @@ __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R
     return true;
 }
 
-// Bond on its owner i: d = x_j - x_i, F_i = K (1 - r0/r) d (src/compute_bonded.h:112-115).
 template <typename R, bool ENERGY>
 __device__ __forceinline__ void bond_ij(const FFS<R> &t, int bt, R dx, R dy, R dz, Acc<R> &acc) {
     const R r2 = dx * dx + dy * dy + dz * dz;
@@ -169,12 +185,7 @@ __device__ inline long long block_index(int swizzle) {
     return (xcd < r) ? (long long)xcd * (q + 1) + k : (long long)r * (q + 1) + (long long)(xcd - r) * q + k;
 }
 
-struct AnchorI {
-    int x, y, z;
-};
-__device__ inline AnchorI unpack(uint32_t w, const Anchor &a) {
-    return AnchorI{(int)(w & a.mx), (int)((w >> a.shy) & a.my), (int)(w >> a.shz)};
-}
+__device__ inline AnchorI unpack(uint32_t w, const Anchor &a) { return anchor_unpack(w, a); }
 
 template <typename R>
 __device__ inline void add_to(Acc<double> &o, const Acc<R> &a) {
@@ -200,38 +211,55 @@ constexpr int UNROLL = ORBC_STEP_UNROLL;
 #endif
 
 // ---- per-particle force accumulation ---------------------------------------
+// The list is walked in groups of UNROLL entries, loading the next group's
+// indices together with this group's gathers; a group is padded with the
+// particle itself (r2 = 0, no contribution).
+
 // Mixed: FP32 pair terms from the FP32 mirrors, summed in FP32 over each group
-// of UNROLL neighbours and in FP64 across groups.
+// and in FP64 across groups.
+template <bool ENERGY>
+__device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long long stride, const PQ *pq,
+                                            const uint32_t *nb, const Anchor &ag, const AnchorI &ai, float4 xi,
+                                            float4 ni, int ti, const FFS<float> &t, Acc<double> &out) {
+    uint32_t jn[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) jn[u] = (kb + u < ke) ? nb[(long long)(kb + u) * stride] : (uint32_t)i;
+    for (int k0 = kb; k0 < ke; k0 += UNROLL) {
+        uint32_t jj[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            jj[u] = jn[u];
+            const int kn = k0 + UNROLL + u;
+            jn[u] = (kn < ke) ? nb[(long long)kn * stride] : (uint32_t)i;
+        }
+        float4 xj[UNROLL], nj[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) { xj[u] = pq[jj[u]].x; nj[u] = pq[jj[u]].n; }
+        Acc<float> acc;
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const AnchorI aj = unpack(__float_as_uint(xj[u].w), ag);
+            const float dx = add_rn(sub_rn(xi.x, xj[u].x), anchor_dx(ag, 0, ai.x - aj.x));
+            const float dy = add_rn(sub_rn(xi.y, xj[u].y), anchor_dx(ag, 1, ai.y - aj.y));
+            const float dz = add_rn(sub_rn(xi.z, xj[u].z), anchor_dx(ag, 2, ai.z - aj.z));
+            const float r2 = r2_rn(dx, dy, dz);
+            pair_ij<float, ENERGY>(t, ti, __float_as_int(nj[u].w), dx, dy, dz, r2, ni.x, ni.y, ni.z, nj[u].x,
+                                   nj[u].y, nj[u].z, acc);
+        }
+        add_to(out, acc);
+    }
+}
+
 template <bool ENERGY>
 __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, const PQ *pq,
-                                                 const uint32_t *nbr, const uint32_t *nnbr, const uint32_t *bnd,
-                                                 const uint8_t *nbnd, const Anchor &ag, float4 xi, float4 ni,
-                                                 const FFS<float> &t, Acc<double> &out, bool pair, bool bond) {
+                                                 const uint32_t *nbr, const uint32_t *nnbr,
+                                                 const uint32_t *bnd, const uint8_t *nbnd, const Anchor &ag, float4 xi,
+                                                 float4 ni, const FFS<float> &t, Acc<double> &out, bool pair,
+                                                 bool bond) {
     const int ti = __float_as_int(ni.w);
     const AnchorI ai = unpack(__float_as_uint(xi.w), ag);
     if (pair) {
-        const int cnt = (int)nnbr[i];
-        const uint32_t *nb = nbr + i;
-        for (int k0 = 0; k0 < cnt; k0 += UNROLL) {
-            uint32_t jj[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) jj[u] = (k0 + u < cnt) ? nb[(long long)(k0 + u) * stride] : (uint32_t)i;
-            float4 xj[UNROLL], nj[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) { xj[u] = pq[jj[u]].x; nj[u] = pq[jj[u]].n; }
-            Acc<float> acc;
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const AnchorI aj = unpack(__float_as_uint(xj[u].w), ag);
-                const float dx = add_rn(sub_rn(xi.x, xj[u].x), ag.h * (float)(ai.x - aj.x));
-                const float dy = add_rn(sub_rn(xi.y, xj[u].y), ag.h * (float)(ai.y - aj.y));
-                const float dz = add_rn(sub_rn(xi.z, xj[u].z), ag.h * (float)(ai.z - aj.z));
-                const float r2 = r2_rn(dx, dy, dz);     // the self entry padding a group gives r2 = 0
-                pair_ij<float, ENERGY>(t, ti, __float_as_int(nj[u].w), dx, dy, dz, r2, ni.x, ni.y, ni.z, nj[u].x,
-                                       nj[u].y, nj[u].z, acc);
-            }
-            add_to(out, acc);
-        }
+        range_mixed<ENERGY>(i, 0, (int)nnbr[i], stride, pq, nbr + i, ag, ai, xi, ni, ti, t, out);
     }
     if (bond && nbnd) {
         const int nb = nbnd[i];
@@ -241,9 +269,9 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
             const uint32_t j = e & BOND_INDEX_MASK;
             const float4 xj = pq[j].x;
             const AnchorI aj = unpack(__float_as_uint(xj.w), ag);
-            const float dx = (xj.x - xi.x) + ag.h * (float)(aj.x - ai.x);
-            const float dy = (xj.y - xi.y) + ag.h * (float)(aj.y - ai.y);
-            const float dz = (xj.z - xi.z) + ag.h * (float)(aj.z - ai.z);
+            const float dx = (xj.x - xi.x) + anchor_dx(ag, 0, aj.x - ai.x);
+            const float dy = (xj.y - xi.y) + anchor_dx(ag, 1, aj.y - ai.y);
+            const float dz = (xj.z - xi.z) + anchor_dx(ag, 2, aj.z - ai.z);
             bond_ij<float, ENERGY>(t, (int)(e >> 30), dx, dy, dz, acc);
         }
         add_to(out, acc);
@@ -251,43 +279,63 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
 }
 
 template <bool ENERGY>
+__device__ __forceinline__ void range_fp64(long long i, int kb, int ke, long long stride, const double *x,
+                                           const double *nrm, const uint8_t *type, const uint32_t *nb,
+                                           const Anchor &ag, bool periodic, double xi,
+                                           double yi, double zi, double nix, double niy, double niz, int ti,
+                                           const FFS<double> &t, Acc<double> &acc) {
+    long long jn[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) jn[u] = (kb + u < ke) ? (long long)nb[(long long)(kb + u) * stride] : i;
+    for (int k0 = kb; k0 < ke; k0 += UNROLL) {
+        long long jj[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            jj[u] = jn[u];
+            const int kn = k0 + UNROLL + u;
+            jn[u] = (kn < ke) ? (long long)nb[(long long)kn * stride] : i;
+        }
+        double xj[UNROLL][3], nj[UNROLL][3];
+        int tj[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            for (int d = 0; d < 3; ++d) { xj[u][d] = x[3 * jj[u] + d]; nj[u][d] = nrm[3 * jj[u] + d]; }
+            tj[u] = type[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            double dx = sub_rn(xi, xj[u][0]);
+            double dy = sub_rn(yi, xj[u][1]);
+            double dz = sub_rn(zi, xj[u][2]);
+            if (periodic) min_image(ag, dx, dy, dz);
+            const double r2 = r2_rn(dx, dy, dz);
+            pair_ij<double, ENERGY>(t, ti, tj[u], dx, dy, dz, r2, nix, niy, niz, nj[u][0], nj[u][1], nj[u][2], acc);
+        }
+    }
+}
+
+template <bool ENERGY>
 __device__ __forceinline__ void accumulate_fp64(long long i, long long stride, const double *x, const double *nrm,
                                                 const uint8_t *type, const uint32_t *nbr, const uint32_t *nnbr,
-                                                const uint32_t *bnd, const uint8_t *nbnd, const FFS<double> &t,
-                                                Acc<double> &acc, bool pair, bool bond) {
+                                                const uint32_t *bnd, const uint8_t *nbnd,
+                                                const Anchor &ag, const FFS<double> &t, Acc<double> &acc, bool pair,
+                                                bool bond) {
     const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
     if (pair) {
         const double nix = nrm[3 * i], niy = nrm[3 * i + 1], niz = nrm[3 * i + 2];
         const int ti = type[i];
-        const int cnt = (int)nnbr[i];
-        const uint32_t *nb = nbr + i;
-        for (int k0 = 0; k0 < cnt; k0 += UNROLL) {
-            long long jj[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) jj[u] = (k0 + u < cnt) ? (long long)nb[(long long)(k0 + u) * stride] : i;
-            double xj[UNROLL][3], nj[UNROLL][3];
-            int tj[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                for (int d = 0; d < 3; ++d) { xj[u][d] = x[3 * jj[u] + d]; nj[u][d] = nrm[3 * jj[u] + d]; }
-                tj[u] = type[jj[u]];
-            }
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const double dx = sub_rn(xi, xj[u][0]);
-                const double dy = sub_rn(yi, xj[u][1]);
-                const double dz = sub_rn(zi, xj[u][2]);
-                const double r2 = r2_rn(dx, dy, dz);
-                pair_ij<double, ENERGY>(t, ti, tj[u], dx, dy, dz, r2, nix, niy, niz, nj[u][0], nj[u][1], nj[u][2], acc);
-            }
-        }
+        const bool periodic = ag.L[0] > 0 || ag.L[1] > 0 || ag.L[2] > 0;
+        range_fp64<ENERGY>(i, 0, (int)nnbr[i], stride, x, nrm, type, nbr + i, ag, periodic, xi, yi, zi, nix, niy, niz,
+                           ti, t, acc);
     }
     if (bond && nbnd) {
         const int nb = nbnd[i];
         for (int k = 0; k < nb; ++k) {
             const uint32_t e = bnd[(long long)k * stride + i];
             const long long j = e & BOND_INDEX_MASK;
-            bond_ij<double, ENERGY>(t, (int)(e >> 30), x[3 * j] - xi, x[3 * j + 1] - yi, x[3 * j + 2] - zi, acc);
+            double dx = x[3 * j] - xi, dy = x[3 * j + 1] - yi, dz = x[3 * j + 2] - zi;
+            min_image(ag, dx, dy, dz);
+            bond_ij<double, ENERGY>(t, (int)(e >> 30), dx, dy, dz, acc);
         }
     }
 }
@@ -301,13 +349,13 @@ __global__ __launch_bounds__(BLOCK) void k_forces(ForceArgs a, int pair, int bon
     if (i >= a.n) return;
     Acc<double> acc;
     if constexpr (MIXED) {
-        accumulate_mixed<ENERGY>(i, a.stride, a.pq, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, a.pq[i].x, a.pq[i].n,
-                                 t, acc, pair, bond);
+        accumulate_mixed<ENERGY>(i, a.stride, a.pq, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, a.pq[i].x,
+                                 a.pq[i].n, t, acc, pair, bond);
         a.f32[3 * i] += (float)acc.fx; a.f32[3 * i + 1] += (float)acc.fy; a.f32[3 * i + 2] += (float)acc.fz;
         a.t32[3 * i] += (float)acc.tx; a.t32[3 * i + 1] += (float)acc.ty; a.t32[3 * i + 2] += (float)acc.tz;
     } else {
         accumulate_fp64<ENERGY>(i, a.stride, a.x, a.nrm, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
-                                t, acc, pair, bond);
+                                a.anc_grid, t, acc, pair, bond);
         a.f64[3 * i] += acc.fx; a.f64[3 * i + 1] += acc.fy; a.f64[3 * i + 2] += acc.fz;
         a.t64[3 * i] += acc.tx; a.t64[3 * i + 1] += acc.ty; a.t64[3 * i + 2] += acc.tz;
     }
@@ -329,16 +377,17 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_export(ForceArgs a, unsigned *i
         if (MIXED) {
             const float4 xi = a.pq[i].x, xj = a.pq[j].x;
             const AnchorI ai = unpack(__float_as_uint(xi.w), a.anc_grid), aj = unpack(__float_as_uint(xj.w), a.anc_grid);
-            const float dx = add_rn(sub_rn(xi.x, xj.x), a.anc_grid.h * (float)(ai.x - aj.x));
-            const float dy = add_rn(sub_rn(xi.y, xj.y), a.anc_grid.h * (float)(ai.y - aj.y));
-            const float dz = add_rn(sub_rn(xi.z, xj.z), a.anc_grid.h * (float)(ai.z - aj.z));
+            const float dx = add_rn(sub_rn(xi.x, xj.x), anchor_dx(a.anc_grid, 0, ai.x - aj.x));
+            const float dy = add_rn(sub_rn(xi.y, xj.y), anchor_dx(a.anc_grid, 1, ai.y - aj.y));
+            const float dz = add_rn(sub_rn(xi.z, xj.z), anchor_dx(a.anc_grid, 2, ai.z - aj.z));
             const float r2 = r2_rn(dx, dy, dz);
             const int kk = __float_as_int(a.pq[i].n.w) * NT + __float_as_int(a.pq[j].n.w);
             hit = r2 > 1e-5f && (r2 < c_fff.cut2_poly[kk] || r2 < c_fff.cut2_lj[kk]);
         } else {
-            const double dx = sub_rn(a.x[3 * i], a.x[3 * (long long)j]);
-            const double dy = sub_rn(a.x[3 * i + 1], a.x[3 * (long long)j + 1]);
-            const double dz = sub_rn(a.x[3 * i + 2], a.x[3 * (long long)j + 2]);
+            double dx = sub_rn(a.x[3 * i], a.x[3 * (long long)j]);
+            double dy = sub_rn(a.x[3 * i + 1], a.x[3 * (long long)j + 1]);
+            double dz = sub_rn(a.x[3 * i + 2], a.x[3 * (long long)j + 2]);
+            min_image(a.anc_grid, dx, dy, dz);
             const double r2 = r2_rn(dx, dy, dz);
             const int kk = a.type[i] * NT + a.type[j];
             hit = r2 > 1e-5 && (r2 < c_ffd.cut2_poly[kk] || r2 < c_ffd.cut2_lj[kk]);
@@ -400,7 +449,7 @@ __global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs
             wi = __float_as_uint(xi4.w);
         } else {
             accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
-                                   tab, acc, true, a.has_bonds != 0);
+                                   a.anc_grid, tab, acc, true, a.has_bonds != 0);
             ti = a.type[i];
             wi = a.anc[i];
         }
@@ -431,7 +480,7 @@ __global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs
         const double u = 4.656612873077392578125e-10;
         const double rx = (double)c0 * u - 1.0, ry = (double)c1 * u - 1.0, rz = (double)c2 * u - 1.0;
         double vx = a.v[3 * i], vy = a.v[3 * i + 1], vz = a.v[3 * i + 2];
-        const double gam = a.gamma[ti], sig = a.sigma[ti];
+        const double gam = a.gamma[ti], sig = a.sig_group ? a.sig_group[a.grp[i] * NT + ti] : a.sigma[ti];
         fx -= gam * vx + sig * rx;
         fy -= gam * vy + sig * ry;
         fz -= gam * vz + sig * rz;
@@ -444,9 +493,9 @@ __global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs
         a.o[3 * i] = ox; a.o[3 * i + 1] = oy; a.o[3 * i + 2] = oz;
         // FP32 mirror with the build-time anchor, and the skin check
         const AnchorI ai = unpack(wi, a.anc_grid);
-        const float r0 = __double2float_rn(px - a.anc_grid.hd * (double)(ai.x + a.anc_grid.A0[0]));
-        const float r1 = __double2float_rn(py - a.anc_grid.hd * (double)(ai.y + a.anc_grid.A0[1]));
-        const float r2 = __double2float_rn(pz - a.anc_grid.hd * (double)(ai.z + a.anc_grid.A0[2]));
+        const float r0 = __double2float_rn(px - anchor_pos(a.anc_grid, 0, ai.x));
+        const float r1 = __double2float_rn(py - anchor_pos(a.anc_grid, 1, ai.y));
+        const float r2 = __double2float_rn(pz - anchor_pos(a.anc_grid, 2, ai.z));
         if (MIXED) {
             PQ q;
             q.x = make_float4(r0, r1, r2, __uint_as_float(wi));
@@ -487,7 +536,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_nh_second(StepArgs a, double *f6
             ti = __float_as_int(qi.n.w);
         } else {
             accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
-                                   tab, acc, true, a.has_bonds != 0);
+                                   a.anc_grid, tab, acc, true, a.has_bonds != 0);
             ti = a.type[i];
         }
         fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;

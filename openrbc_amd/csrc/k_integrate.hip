@@ -68,11 +68,10 @@ __device__ inline void write_mirror(const IntegArgs &a, long long i, const doubl
     if (!a.pq) return;
     const uint32_t w = a.anc[i];
     const Anchor &g = a.anc_grid;
-    const long long A[3] = {(long long)(w & g.mx) + g.A0[0], (long long)((w >> g.shy) & g.my) + g.A0[1],
-                            (long long)(w >> g.shz) + g.A0[2]};
+    const long long A[3] = {(long long)(w & g.mx), (long long)((w >> g.shy) & g.my), (long long)(w >> g.shz)};
     PQ q;
-    q.x = make_float4(__double2float_rn(x[0] - g.hd * (double)A[0]), __double2float_rn(x[1] - g.hd * (double)A[1]),
-                      __double2float_rn(x[2] - g.hd * (double)A[2]), __uint_as_float(w));
+    q.x = make_float4(__double2float_rn(x[0] - anchor_pos(g, 0, A[0])), __double2float_rn(x[1] - anchor_pos(g, 1, A[1])),
+                      __double2float_rn(x[2] - anchor_pos(g, 2, A[2])), __uint_as_float(w));
     q.n = make_float4(__double2float_rn(n[0]), __double2float_rn(n[1]), __double2float_rn(n[2]), __int_as_float(ty));
     a.pq[i] = q;
 }
@@ -116,7 +115,7 @@ __global__ __launch_bounds__(BLOCK) void k_langevin(IntegArgs a) {
     const double r[3] = {(double)c0 * u - 1.0, (double)c1 * u - 1.0, (double)c2 * u - 1.0};
     const double dtm = a.dt * c_inv_mass[ty];
     for (int d = 0; d < 3; ++d) {
-        f[d] -= a.gamma[ty] * v[d] + a.sigma[ty] * r[d];
+        f[d] -= a.gamma[ty] * v[d] + (a.sig_group ? a.sig_group[a.grp[i] * NT + ty] : a.sigma[ty]) * r[d];
         v[d] += f[d] * dtm;
         x[d] += v[d] * a.dt;
     }
@@ -143,6 +142,7 @@ __global__ __launch_bounds__(BLOCK) void k_nh_first(IntegArgs a) {
             x[d] += v[d] * a.dt;
         }
         for (int d = 0; d < 3; ++d) {
+            if (a.anc_grid.L[d] > 0) continue;  // periodic axis: no walls
             if (x[d] < a.box_lo[d]) { x[d] = a.box_lo[d] + (a.box_lo[d] - x[d]); v[d] = -v[d]; }
             else if (x[d] > a.box_hi[d]) { x[d] = a.box_hi[d] - (x[d] - a.box_hi[d]); v[d] = -v[d]; }
         }
@@ -198,6 +198,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(IntegArgs a) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= a.n) return;
     for (int d = 0; d < 3; ++d) {
+        if (a.anc_grid.L[d] > 0) continue;      // periodic axis: no walls
         double x = a.x[3 * i + d];
         if (x < a.box_lo[d]) { a.x[3 * i + d] = a.box_lo[d] + (a.box_lo[d] - x); a.v[3 * i + d] = -a.v[3 * i + d]; }
         else if (x > a.box_hi[d]) { a.x[3 * i + d] = a.box_hi[d] - (x - a.box_hi[d]); a.v[3 * i + d] = -a.v[3 * i + d]; }
@@ -309,6 +310,13 @@ __global__ __launch_bounds__(BLOCK) void k_gather3f(long long n, const uint32_t 
     const long long s = slot[i];
     for (int d = 0; d < 3; ++d) dst[3 * i + d] = (float)src[3 * s + d];
 }
+// dst[i] = src[slot[i]]: a per-particle u16 array from upload order to device order
+__global__ __launch_bounds__(BLOCK) void k_gather_u16(long long n, const uint32_t *slot, const uint16_t *src,
+                                                      uint16_t *dst) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) dst[i] = src[slot[i]];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t v, long long n, Gate gt) {
     if (gate_closed(gt)) return;
     for (long long i = (long long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long long)gridDim.x * BLOCK) p[i] = v;
@@ -390,6 +398,10 @@ void launch_gather_slots(long long n, const uint32_t *slot, const double *src3, 
 void launch_gather_slots_f(long long n, const uint32_t *slot, const double *src3, float *dst3, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_gather3f, dim3(nblk(n)), dim3(BLOCK), 0, s, n, slot, src3, dst3);
 }
+void launch_gather_u16(long long n, const uint32_t *slot, const uint16_t *src, uint16_t *dst, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_u16, dim3(nblk(n)), dim3(BLOCK), 0, s, n, slot, src, dst);
+}
+
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s, Gate gt) {
     if (n <= 0) return;
     long long nb = nblk(n);

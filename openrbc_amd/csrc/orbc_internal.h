@@ -40,19 +40,54 @@ double ff_max_cut(int a, int b);                            // rc(a,b), 0 if non
 // Geometry of the FP32 mirrors: position = h * A + r, A an integer anchor per
 // axis fixed at the last neighbour build, r the FP32 offset.  A - A0 is packed
 // into 32 bits (bits[0] | bits[1] << shy | bits[2] << shz).
+// Packed 32-bit integer anchor of each particle; the FP32 mirror stores the
+// offset r = x - anchor position.  Per axis d the position of anchor A is
+// lo[d] + hd[d] * (A + A0[d]).  Open axes: lo = 0 and hd a power of two, so
+// hf * dA is exact in FP32.  Periodic axes: A0 = 0, lo = box origin,
+// hd = L / 2^bits, and anchor differences wrap to the nearest image by sign
+// extension from `bits` (wrap = 32 - bits; 0 on open axes).
 struct Anchor {
-    double hd;       // anchor spacing (power of two)
-    float h;
-    long long A0[3]; // anchor origin
+    double hd[3];    // anchor spacing
+    float hf[3];
+    double lo[3];
+    long long A0[3]; // anchor origin (open axes)
     unsigned shy, shz, mx, my, mz;
+    unsigned wrap[3];
+    double L[3], Linv[3];   // periodic axes: box length (FP64 minimum image); 0 on open axes
 };
 
-// Uniform binning grid of the neighbour build (Morton-ordered cells).
+struct AnchorI {
+    int x, y, z;
+};
+__device__ inline AnchorI anchor_unpack(uint32_t w, const Anchor &a) {
+    return AnchorI{(int)(w & a.mx), (int)((w >> a.shy) & a.my), (int)(w >> a.shz)};
+}
+// anchor difference to the nearest image (identity on open axes)
+__device__ inline int anchor_wrap(int dA, unsigned s) { return (int)((unsigned)dA << s) >> s; }
+// FP32 distance between two anchors along one axis
+__device__ inline float anchor_dx(const Anchor &a, int axis, int dA) {
+    return a.hf[axis] * (float)anchor_wrap(dA, a.wrap[axis]);
+}
+// FP64 position of anchor A along one axis
+__device__ inline double anchor_pos(const Anchor &a, int axis, long long A) {
+    return a.lo[axis] + a.hd[axis] * (double)(A + a.A0[axis]);
+}
+// FP64 minimum image on the periodic axes
+__device__ inline void min_image(const Anchor &a, double &dx, double &dy, double &dz) {
+    if (a.L[0] > 0) dx -= a.L[0] * rint(dx * a.Linv[0]);
+    if (a.L[1] > 0) dy -= a.L[1] * rint(dy * a.Linv[1]);
+    if (a.L[2] > 0) dz -= a.L[2] * rint(dz * a.Linv[2]);
+}
+
+// Uniform binning grid of the neighbour build (Morton-ordered cells).  On a
+// periodic axis the cells tile [lo, lo + L) exactly and neighbour cells wrap.
 struct Grid {
     double lo[3];
-    double cs;       // cell edge
+    double cs;       // nominal cell edge
+    double csd[3];   // cell edge per axis (>= cs; L / dim on periodic axes)
     int dim[3];
     int bits[3];     // Morton bits per axis
+    int per[3];      // periodic axis
     long long ncell;
 };
 
@@ -105,6 +140,10 @@ struct StepArgs {
     Anchor anc_grid;
     double dt;
     double gamma[NT], sigma[NT];
+    // ensemble members with their own temperature: sigma of particle i is
+    // sig_group[grp[i] * NT + type]; null: sigma[type]
+    const uint16_t *grp;
+    const double *sig_group;
     unsigned long long seed, step;
     // skin criterion on the device: step k reads slot (k+2)%3 (displacement of
     // its input positions), writes slot k%3 (of its output) and clears slot
@@ -149,6 +188,8 @@ void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
 
 // k_build.hip
 void launch_bbox(const double *x, long long n, double *out6, hipStream_t s);
+// periodic axes: x -> lo + (x - lo) - L floor((x - lo) / L)
+void launch_wrap(double *x, long long n, const double lo[3], const double L[3], const int per[3], hipStream_t s);
 void launch_keys(const double *x, long long n, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s);
 struct PermuteArgs {
     long long n;
@@ -159,6 +200,7 @@ struct PermuteArgs {
     const uint32_t *id_src; uint32_t *id_dst;
     const uint32_t *slot_src; uint32_t *slot_dst;
     const int32_t *aff_src; int32_t *aff_dst;
+    const uint16_t *grp_src; uint16_t *grp_dst;         // ensemble member (or null)
 };
 void launch_permute(const PermuteArgs &a, hipStream_t s);
 void launch_inverse(const uint32_t *slot, uint32_t *inv, long long n, hipStream_t s);
@@ -183,7 +225,8 @@ struct NlistArgs {
 };
 long long launch_cell_list(const double *x, long long n, Grid g, uint32_t *flag, uint32_t *pos, uint32_t *occ,
                            void *scan_temp, hipStream_t s);
-void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, hipStream_t s);
+// far: some type reaches past the first ring of cells (outer shells appended by a second kernel)
+void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool far, hipStream_t s);
 void set_list_cutoffs(const float *cut2_list, const int *rings, hipStream_t s);
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g,
                  unsigned *disp_max, hipStream_t s);
@@ -213,6 +256,8 @@ struct IntegArgs {
     Anchor anc_grid;
     double dt;
     double gamma[NT], sigma[NT];
+    const uint16_t *grp;      // ensemble member of each particle, or null (see StepArgs)
+    const double *sig_group;
     unsigned long long seed, step;
     double box_lo[3], box_hi[3];
     const double *zeta;       // device scalar
@@ -234,6 +279,7 @@ void launch_scatter_slots1(long long n, const uint32_t *slot, const double *src,
 void launch_gather_slots(long long n, const uint32_t *slot, const double *src3, double *dst3, hipStream_t s);
 void launch_gather_slots_f(long long n, const uint32_t *slot, const double *src3, float *dst3, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s, Gate gt = {});
+void launch_gather_u16(long long n, const uint32_t *slot, const uint16_t *src, uint16_t *dst, hipStream_t s);
 
 // k_voronoi.hip
 struct VoroArgs {

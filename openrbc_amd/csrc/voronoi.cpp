@@ -30,6 +30,7 @@ Grid make_grid(const double lo[3], const double hi[3], double cs) {
     Grid g{};
     g.cs = cs;
     for (int d = 0; d < 3; ++d) {
+        g.csd[d] = cs;
         g.lo[d] = lo[d];
         g.dim[d] = std::max(1, (int)std::ceil((hi[d] - lo[d]) / cs) + 1);
         g.bits[d] = ceil_log2(g.dim[d]);
@@ -79,7 +80,7 @@ void index_centroids(orbc_ctx *c) {
     double lo[3], hi[3];
     for (int d = 0; d < 3; ++d) {
         lo[d] = c->grid.lo[d];
-        hi[d] = c->grid.lo[d] + c->grid.cs * c->grid.dim[d];
+        hi[d] = c->grid.lo[d] + c->grid.csd[d] * c->grid.dim[d];
     }
     V.cg = make_grid(lo, hi, kCentroidCell);
     if (V.cg.ncell > V.cg_cap) {
@@ -110,6 +111,8 @@ void update_centroids(orbc_ctx *c) {
 }
 
 void voro_alloc(orbc_ctx *c, long long ncell) {
+    for (int d = 0; d < 3; ++d)
+        if (c->cfg.periodic[d]) throw std::runtime_error("Voronoi cells are not defined on a periodic box");
     voronoi_free(c);
     c->vol_target_set = false;
     auto &V = c->voro;

@@ -10,6 +10,7 @@
   openrbc_amd/data/rbc_example_large.npz.
 * ``random_sphere``: init_random_sphere (src/init_random.h:27-66), lipids only.
 * ``planar_patch``: hexagonal lipid lattice (periodic patch workloads).
+* ``vesicle_ensemble``: many lipid vesicles of uneven size (batched ensembles).
 """
 from __future__ import annotations
 
@@ -184,6 +185,38 @@ def random_sphere(r: float = 100.0, rho: float = 1.05, seed: int = 0):
     x = np.stack([r * s * np.cos(theta), r * s * np.sin(theta), r * u], 1)
     return dict(lipid_x=x, lipid_n=x / np.linalg.norm(x, axis=1, keepdims=True),
                 lipid_id=np.arange(1, n + 1, dtype=np.uint32))
+
+
+def vesicle_ensemble(n_members: int = 64, n_min: int = 5000, n_max: int = 300000, rho: float = 1.05,
+                     kT: float = 0.22, spread: float = 0.1, gap: float = 12.0, seed: int = 0):
+    """Batched-ensemble workload: lipid-only vesicles (the init_random_sphere
+    rule, src/init_random.h:27-66, uniform points on a sphere, directors
+    radial) with particle counts log-uniform in [n_min, n_max], member
+    temperatures kT * U(1 - spread, 1 + spread), centred on a cubic lattice
+    whose spacing leaves `gap` between the largest vesicles.  Ids are unique
+    across members.  Returns lipid_x, lipid_n, lipid_id, group (member of each
+    lipid), kT (per member), radius and centre (per member)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_members)).astype(np.int64)
+    radius = np.sqrt(counts / (4 * np.pi * rho))
+    kts = kT * rng.uniform(1 - spread, 1 + spread, n_members)
+    side = int(np.ceil(n_members ** (1 / 3)))
+    pitch = 2 * radius.max() + gap
+    xs, ns, gs, centres = [], [], [], []
+    for m in range(n_members):
+        c = pitch * np.array([m % side, (m // side) % side, m // (side * side)], np.float64)
+        n = int(counts[m])
+        u = 2 * rng.random(n) - 1
+        phi = 2 * np.pi * rng.random(n)
+        s = np.sqrt(1 - u * u)
+        d = np.stack([s * np.cos(phi), s * np.sin(phi), u], 1)
+        xs.append(c + radius[m] * d)
+        ns.append(d)
+        gs.append(np.full(n, m, np.int32))
+        centres.append(c)
+    x = np.vstack(xs)
+    return dict(lipid_x=x, lipid_n=np.vstack(ns), lipid_id=np.arange(1, len(x) + 1, dtype=np.uint32),
+                group=np.concatenate(gs), kT=kts, radius=radius, centre=np.array(centres))
 
 
 def planar_patch(nx: int, ny: int, a: float = 1.0487, tilt: float = 0.02, jitter: float = 0.02, seed: int = 0):

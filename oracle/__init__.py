@@ -100,6 +100,37 @@ class Oracle:
         L.orc_pairs_voronoi_fp64.argtypes = [C.c_int64, _d, _i32, _i32, C.c_int64, _f, C.c_int64, _i64]
         L.orc_constrain_volume.restype = C.c_double
         L.orc_constrain_volume.argtypes = [C.c_int64, _d, _i32, _i32, C.c_int64, _f, C.c_double, C.c_double, _d]
+        L.orc_set_box.restype = None
+        L.orc_set_box.argtypes = [_d, _d, _i32]
+        L.orc_wrap.restype = None
+        L.orc_wrap.argtypes = [C.c_int64, _d]
+        L.orc_pairs_brute_mixed_anchor.restype = C.c_int64
+        L.orc_pairs_brute_mixed_anchor.argtypes = [C.c_int64, _d, _d, _i32, _d, _d, _d, _i32, C.c_int64, _i64]
+        self.set_box(None)
+
+    def set_box(self, box):
+        """box = (lo[3], hi[3], periodic[3]) or None (no periodic axis).  All
+        FP64 pair/force/bond functions take minimum images on periodic axes."""
+        if box is None:
+            lo = np.zeros(3); L = np.zeros(3); per = np.zeros(3, np.int32)
+        else:
+            lo = np.asarray(box[0], np.float64); L = np.asarray(box[1], np.float64) - lo
+            per = np.asarray(box[2], np.int32)
+        self.L.orc_set_box(_p(_c(lo, np.float64), _d), _p(_c(L, np.float64), _d), _p(_c(per, np.int32), _i32))
+
+    def wrap(self, x):
+        x = np.array(x, np.float64, order="C", copy=True)
+        self.L.orc_wrap(len(x), _p(x, _d))
+        return x
+
+    def pairs_mixed_anchor(self, x, type_, obs, xa=None):
+        """Mixed pair set with the device's per-axis anchor grid (observables of a Simulation)."""
+        x = _c(x, np.float64); t = _c(type_, np.int32)
+        xa = x if xa is None else _c(xa, np.float64)
+        lo = np.asarray(obs["anchor_lo"], np.float64); h = np.asarray(obs["anchor_spacing"], np.float64)
+        A0 = np.asarray(obs["anchor_origin"], np.float64); bits = np.asarray(obs["anchor_bits"], np.int32)
+        return self._pairs(self.L.orc_pairs_brute_mixed_anchor, len(t), _p(x, _d), _p(xa, _d), _p(t, _i32),
+                           _p(lo, _d), _p(h, _d), _p(A0, _d), _p(bits, _i32))
 
     def ff(self) -> dict:
         """Force-field tables, 6x6 arrays indexed [owner type, neighbour type]."""

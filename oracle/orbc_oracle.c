@@ -179,6 +179,36 @@ int orc_pair(int ti, int tj, const double dx[3], const double ni[3], const doubl
     return 1;
 }
 
+/* ------------------------------------------------------------------------ */
+static struct { double lo[3], L[3]; int per[3]; } BOX;
+
+void orc_set_box(const double lo[3], const double L[3], const int per[3]) {
+    for (int d = 0; d < 3; ++d) {
+        BOX.lo[d] = lo ? lo[d] : 0.0;
+        BOX.L[d] = L ? L[d] : 0.0;
+        BOX.per[d] = per ? (per[d] != 0) : 0;
+    }
+}
+
+static inline void min_image(double dx[3]) {
+    for (int d = 0; d < 3; ++d)
+        if (BOX.per[d]) dx[d] -= BOX.L[d] * rint(dx[d] / BOX.L[d]);
+}
+
+void orc_wrap(int64_t n, double *x) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d)
+            if (BOX.per[d]) {
+                const double u = x[3 * i + d] - BOX.lo[d];
+                x[3 * i + d] = BOX.lo[d] + (u - BOX.L[d] * floor(u / BOX.L[d]));
+            }
+}
+
+static inline void pair_dx(const double *x, int64_t a, int64_t b, double dx[3]) {
+    for (int d = 0; d < 3; ++d) dx[d] = x[3 * a + d] - x[3 * b + d];
+    min_image(dx);
+}
+
 static inline int interacts_fp64(int ti, int tj, double r2) {
     const orc_ff *ff = orc_ff_get();
     const int k = ti * 6 + tj;
@@ -191,7 +221,8 @@ int64_t orc_pairs_brute_fp64(int64_t n, const double *x, const int32_t *type,
     int64_t cnt = 0;
     for (int64_t i = 0; i < n; ++i)
         for (int64_t j = i + 1; j < n; ++j) {
-            const double dx[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
+            double dx[3];
+            pair_dx(x, i, j, dx);
             if (interacts_fp64(type[i], type[j], dot3(dx, dx))) {
                 if (cnt < cap) { out[2 * cnt] = i; out[2 * cnt + 1] = j; }
                 ++cnt;
@@ -216,8 +247,8 @@ int64_t orc_pairs_cell_fp64(int64_t n, const double *x, const int32_t *type,
         if (ff->cut2_poly[k] > cmax2) cmax2 = ff->cut2_poly[k];
         if (ff->cut2_lj[k] > cmax2) cmax2 = ff->cut2_lj[k];
     }
-    const double cs = sqrt(cmax2) * 1.0001;
-    double lo[3], hi[3];
+    const double cmax = sqrt(cmax2) * 1.0001;
+    double lo[3], hi[3], cs[3];
     for (int d = 0; d < 3; ++d) { lo[d] = x[d]; hi[d] = x[d]; }
     for (int64_t i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) {
@@ -225,14 +256,28 @@ int64_t orc_pairs_cell_fp64(int64_t n, const double *x, const int32_t *type,
             if (x[3 * i + d] > hi[d]) hi[d] = x[3 * i + d];
         }
     int64_t dim[3];
-    for (int d = 0; d < 3; ++d) dim[d] = (int64_t)((hi[d] - lo[d]) / cs) + 1;
+    for (int d = 0; d < 3; ++d) {
+        if (BOX.per[d]) {
+            /* periodic: cells tile [lo, lo+L) exactly (at least 3, else brute force) */
+            lo[d] = BOX.lo[d];
+            dim[d] = (int64_t)floor(BOX.L[d] / cmax);
+            if (dim[d] < 3) return orc_pairs_brute_fp64(n, x, type, cap, out);
+            cs[d] = BOX.L[d] / (double)dim[d];
+        } else {
+            cs[d] = cmax;
+            dim[d] = (int64_t)((hi[d] - lo[d]) / cs[d]) + 1;
+        }
+    }
     const int64_t ncell = dim[0] * dim[1] * dim[2];
     int64_t *cell = malloc(sizeof(int64_t) * n), *start = calloc(ncell + 1, sizeof(int64_t));
     int64_t *order = malloc(sizeof(int64_t) * n), *fill = calloc(ncell, sizeof(int64_t));
     int64_t (*cc)[3] = malloc(sizeof(*cc) * n);
     for (int64_t i = 0; i < n; ++i) {
         for (int d = 0; d < 3; ++d) {
-            int64_t c = (int64_t)((x[3 * i + d] - lo[d]) / cs);
+            double u = x[3 * i + d] - lo[d];
+            if (BOX.per[d]) u -= BOX.L[d] * floor(u / BOX.L[d]);
+            int64_t c = (int64_t)(u / cs[d]);
+            if (c < 0) c = 0;
             if (c >= dim[d]) c = dim[d] - 1;
             cc[i][d] = c;
         }
@@ -246,13 +291,19 @@ int64_t orc_pairs_cell_fp64(int64_t n, const double *x, const int32_t *type,
         for (int64_t dz = -1; dz <= 1; ++dz)
             for (int64_t dy = -1; dy <= 1; ++dy)
                 for (int64_t dxc = -1; dxc <= 1; ++dxc) {
-                    const int64_t c0 = cc[i][0] + dxc, c1 = cc[i][1] + dy, c2 = cc[i][2] + dz;
-                    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= dim[0] || c1 >= dim[1] || c2 >= dim[2]) continue;
-                    const int64_t c = (c2 * dim[1] + c1) * dim[0] + c0;
+                    int64_t c3[3] = {cc[i][0] + dxc, cc[i][1] + dy, cc[i][2] + dz};
+                    int out_of = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        if (BOX.per[d]) c3[d] = (c3[d] + dim[d]) % dim[d];
+                        else if (c3[d] < 0 || c3[d] >= dim[d]) out_of = 1;
+                    }
+                    if (out_of) continue;
+                    const int64_t c = (c3[2] * dim[1] + c3[1]) * dim[0] + c3[0];
                     for (int64_t q = start[c]; q < start[c + 1]; ++q) {
                         const int64_t j = order[q];
                         if (j <= i) continue;
-                        const double dx[3] = {x[3 * i] - x[3 * j], x[3 * i + 1] - x[3 * j + 1], x[3 * i + 2] - x[3 * j + 2]};
+                        double dx[3];
+                        pair_dx(x, i, j, dx);
                         if (interacts_fp64(type[i], type[j], dot3(dx, dx))) {
                             if (cnt < cap) { out[2 * cnt] = i; out[2 * cnt + 1] = j; }
                             ++cnt;
@@ -262,6 +313,47 @@ int64_t orc_pairs_cell_fp64(int64_t n, const double *x, const int32_t *type,
     }
     if (cnt <= cap) qsort(out, (size_t)cnt, 2 * sizeof(int64_t), cmp_pair);
     free(cell); free(start); free(order); free(fill); free(cc);
+    return cnt;
+}
+
+int64_t orc_pairs_brute_mixed_anchor(int64_t n, const double *x, const double *xa, const int32_t *type,
+                                     const double lo[3], const double h[3], const double A0[3],
+                                     const int bits[3], int64_t cap, int64_t *out) {
+    const orc_ff *ff = orc_ff_get();
+    float cut2p[36], cut2l[36];
+    for (int k = 0; k < 36; ++k) { cut2p[k] = (float)ff->cut2_poly[k]; cut2l[k] = (float)ff->cut2_lj[k]; }
+    int64_t *A = malloc(sizeof(int64_t) * 3 * n);
+    float *r = malloc(sizeof(float) * 3 * n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            int64_t a = (int64_t)floor((xa[3 * i + d] - lo[d]) / h[d]) - (int64_t)A0[d];
+            if (bits[d] > 0) {
+                const int64_t M = (int64_t)1 << bits[d];
+                if (a < 0) a = 0;
+                if (a > M - 1) a = M - 1;
+            }
+            A[3 * i + d] = a;
+            r[3 * i + d] = (float)(x[3 * i + d] - (lo[d] + h[d] * (double)(a + (int64_t)A0[d])));
+        }
+    int64_t cnt = 0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = i + 1; j < n; ++j) {
+            float dd[3];
+            for (int k = 0; k < 3; ++k) {
+                int32_t dA = (int32_t)(A[3 * i + k] - A[3 * j + k]);
+                if (bits[k] > 0) dA = (int32_t)((uint32_t)dA << (32 - bits[k])) >> (32 - bits[k]);
+                const float a = r[3 * i + k] - r[3 * j + k];
+                const float s = (float)h[k] * (float)dA;
+                dd[k] = a + s;
+            }
+            const float r2 = fmaf(dd[2], dd[2], fmaf(dd[1], dd[1], dd[0] * dd[0]));
+            const int k = type[i] * 6 + type[j];
+            if (r2 > 1e-5f && (r2 < cut2p[k] || r2 < cut2l[k])) {
+                if (cnt < cap) { out[2 * cnt] = i; out[2 * cnt + 1] = j; }
+                ++cnt;
+            }
+        }
+    free(A); free(r);
     return cnt;
 }
 
@@ -306,7 +398,8 @@ void orc_forces_pairs(int64_t n, const double *x, const double *nrm, const int32
         /* owner-computes both sides: src/compute_pairwise_fused.h:249-316 */
         for (int side = 0; side < 2; ++side) {
             const int64_t a = side ? j : i, b = side ? i : j;
-            const double dx[3] = {x[3 * a] - x[3 * b], x[3 * a + 1] - x[3 * b + 1], x[3 * a + 2] - x[3 * b + 2]};
+            double dx[3];
+            pair_dx(x, a, b, dx);
             double u = 0, ww = 0;
             if (orc_pair(type[a], type[b], dx, nrm + 3 * a, nrm + 3 * b, f + 3 * a, t + 3 * a, &u, &ww)) {
                 if (e) e[a] += 0.5 * u;
@@ -321,7 +414,8 @@ void orc_bonded(int64_t nb, const int32_t *bt, const int64_t *bi, const int64_t 
     const orc_ff *ff = orc_ff_get();
     for (int64_t k = 0; k < nb; ++k) {
         const int64_t i = bi[k], j = bj[k];
-        const double dx[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+        double dx[3];
+        pair_dx(x, j, i, dx);
         const double r = sqrt(dot3(dx, dx));
         const double K = ff->bond_K[bt[k]], r0 = ff->bond_r0[bt[k]];
         const double c = K * (1.0 - r0 / r);

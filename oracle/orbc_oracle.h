@@ -53,6 +53,16 @@ const orc_ff *orc_ff_get(void);
 int orc_pair(int ti, int tj, const double dx[3], const double ni[3], const double nj[3],
              double fi[3], double tq[3], double *u, double *w);
 
+/* ---- periodic box ---------------------------------------------------------
+ * The reference has no periodic boundaries (its box only bounces particles
+ * back, src/integrate_nh.h:124-144).  The planar-patch workload needs them;
+ * this restates the build's definition: on a periodic axis d every pair
+ * vector takes the minimum image dx - L*rint(dx/L), and positions are wrapped
+ * into [lo, lo+L) by x -> lo + (x-lo) - L*floor((x-lo)/L).  All FP64
+ * functions below honour the box set here (default: no periodic axis). */
+void orc_set_box(const double lo[3], const double L[3], const int per[3]);
+void orc_wrap(int64_t n, double *x);
+
 /* ---- pair sets ----------------------------------------------------------- */
 /* Unordered interacting pairs (i<j), FP64 selection 1e-5 < r2 < rc2(ti,tj).
  * Brute force O(N^2).  Returns count; writes up to cap pairs into out[2*k]. */
@@ -65,6 +75,15 @@ int64_t orc_pairs_cell_fp64(int64_t n, const double *x, const int32_t *type,
  * spacing h (power of two): A = floor(x/h), r32 = (float)(x - h*A),
  * dx = (r32_i - r32_j) + h*(A_i - A_j) in FP32, r2 = fma(dz,dz,fma(dy,dy,dx*dx)).
  * Anchors are taken from xa (positions at the last neighbour build). */
+/* Mixed selection with the general anchor scheme of the device, per axis d:
+ * A = floor((xa - lo_d) / h_d) - A0_d, clamped to [0, 2^bits_d - 1] on
+ * periodic axes, r = (float)(x - (lo_d + h_d (A + A0_d))), and
+ * dx = fl(fl(r_i - r_j) + fl((float)h_d * (float)dA)) where on a periodic axis
+ * dA = A_i - A_j sign-extended from bits_d (wrap to the nearest image).
+ * Open axes: lo_d = 0, h_d a power of two, bits_d = 0 (no wrap). */
+int64_t orc_pairs_brute_mixed_anchor(int64_t n, const double *x, const double *xa, const int32_t *type,
+                                     const double lo[3], const double h[3], const double A0[3],
+                                     const int bits[3], int64_t cap, int64_t *out);
 int64_t orc_pairs_brute_mixed(int64_t n, const double *x, const double *xa, const int32_t *type,
                               double h, int64_t cap, int64_t *out);
 

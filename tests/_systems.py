@@ -76,3 +76,43 @@ def split(s):
     # proteins ordered by tag so tag2idx is the identity shift
     prot = prot[np.argsort(s["id"][prot])]
     return lip, prot
+
+
+def periodic_patch(nx=16, ny=16, seed=0, spacing=1.0487, origin=(-3.0, 5.0), tilt=0.05, jitter=0.05):
+    """Hexagonal lipid sheet that tiles a periodic box in x and y (ny even),
+    z open: glycophorin (type 3) and band-3 (type 2) replace sheet sites,
+    actin (type 4) sits 2.25 below each glycophorin with a type-0 bond.
+    Positions are jittered and may fall outside the box (the device wraps
+    them).  Returns the system dict plus box = (lo, hi, periodic)."""
+    assert ny % 2 == 0
+    rng = np.random.default_rng(seed)
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    x = spacing * (ii + 0.5 * (jj % 2)) + origin[0]
+    y = spacing * (np.sqrt(3) / 2) * jj + origin[1]
+    pts = np.stack([x.ravel(), y.ravel(), np.zeros(x.size)], 1)
+    pts += rng.normal(0, jitter * spacing, pts.shape)
+    nrm = np.tile([0.0, 0.0, 1.0], (len(pts), 1)) + rng.normal(0, tilt, (len(pts), 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    types = np.zeros(len(pts), np.int32)
+    sites = rng.choice(len(pts), size=max(2, len(pts) // 30), replace=False)
+    gly, b3 = sites[: len(sites) // 2], sites[len(sites) // 2:]
+    types[gly] = 3
+    types[b3] = 2
+    n0 = len(pts)
+    act = pts[gly] - np.array([0, 0, 2.25]) + rng.normal(0, 0.05, (len(gly), 3))
+    pts = np.vstack([pts, act])
+    nrm = np.vstack([nrm, nrm[gly]])
+    types = np.concatenate([types, np.full(len(gly), 4, np.int32)])
+    bonds = [(0, g, n0 + k) for k, g in enumerate(gly)]
+    n = len(pts)
+    prot = np.where(types != 0)[0]
+    lip = np.where(types == 0)[0]
+    ids = np.zeros(n, np.uint32)
+    ids[prot] = np.arange(1, len(prot) + 1)
+    ids[lip] = np.arange(len(prot) + 1, len(prot) + 1 + len(lip))
+    b = np.array(bonds, dtype=np.int64).reshape(-1, 3)
+    lo = np.array([origin[0], origin[1], -1000.0])
+    hi = np.array([origin[0] + nx * spacing, origin[1] + ny * spacing * np.sqrt(3) / 2, 1000.0])
+    return dict(x=pts, n=nrm, v=rng.normal(0, 0.3, (n, 3)), o=rng.normal(0, 0.1, (n, 3)), type=types, id=ids,
+                bonds=(b[:, 0].astype(np.int32), b[:, 1], b[:, 2]),
+                box=(lo, hi, np.array([1, 1, 0], np.int32)))

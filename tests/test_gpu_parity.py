@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import openrbc_amd as orbc
-from _systems import patch
+from _systems import patch, periodic_patch
 
 pytestmark = pytest.mark.gpu
 
@@ -457,3 +457,209 @@ def test_partition_exact_with_drifted_centroid_index(oracle):
     c, al, _ = sim.voronoi_get()
     x = sim.get_state(orbc.LIPID)["x"]
     assert np.array_equal(al, oracle.partition(x, c))
+
+
+# ---- periodic boxes (planar patch workloads; no reference counterpart) ----------------
+@pytest.fixture(scope="module", params=[0, 7])
+def psystem(request):
+    return periodic_patch(18, 16, seed=request.param)
+
+
+def make_psim(s, precision, **kw):
+    lo, hi, per = s["box"]
+    return make_sim(s, precision, box_lo=list(lo), box_hi=list(hi), periodic=[int(p) for p in per], **kw)
+
+
+@pytest.fixture
+def pbox(oracle):
+    yield oracle
+    oracle.set_box(None)
+
+
+@pytest.mark.parametrize("precision", [orbc.FP64, orbc.MIXED])
+def test_periodic_pair_set_bit_exact(psystem, precision, pbox):
+    s = psystem
+    sim, lip, prot = make_psim(s, precision)
+    x = gather(sim, lip, prot, "state")["x"]          # wrapped into the box by the build
+    lo, hi, per = s["box"]
+    assert np.all(x[:, :2] >= lo[:2]) and np.all(x[:, :2] < hi[:2])
+    pbox.set_box(s["box"])
+    if precision == orbc.FP64:
+        exp = id_pairs(s, pbox.pairs_brute(x, s["type"]))
+    else:
+        exp = id_pairs(s, pbox.pairs_mixed_anchor(x, s["type"], sim.observables()))
+    got = sim.pairs()
+    assert len(got) == len(exp) > 0 and np.array_equal(got, exp)
+    pbox.set_box(None)
+    assert len(id_pairs(s, pbox.pairs_brute(x, s["type"]))) < len(got)    # pairs across the edges exist
+
+
+@pytest.mark.parametrize("precision,tol", [(orbc.FP64, EPS_FP64), (orbc.MIXED, EPS_MIXED)])
+def test_periodic_forces_vs_oracle(psystem, precision, tol, pbox):
+    s = psystem
+    sim, lip, prot = make_psim(s, precision)
+    sim.compute_pairwise(zero_first=True, energy=True)
+    sim.compute_bonded(energy=True)
+    f, t, e, w = gather(sim, lip, prot, energy=True)
+    x = gather(sim, lip, prot, "state")["x"]
+    pbox.set_box(s["box"])
+    of, ot, oe, ow = pbox.forces(x, s["n"], s["type"], bonds=s["bonds"])
+    assert eps_f(f, of) < tol
+    assert eps_f(t, ot) < tol
+    assert np.abs(e - oe).max() / np.sqrt(np.mean(oe ** 2)) < 10 * tol
+
+
+@pytest.mark.parametrize("precision,tol", [(orbc.FP64, 1e-10), (orbc.MIXED, 2e-5)])
+def test_periodic_translation_invariance(psystem, precision, tol):
+    s = psystem
+    lo, hi, _ = s["box"]
+    out = []
+    for shift in (np.zeros(3), np.array([0.37 * (hi[0] - lo[0]), 0.61 * (hi[1] - lo[1]), 0.0])):
+        t = dict(s)
+        t["x"] = s["x"] + shift
+        sim, lip, prot = make_psim(t, precision)
+        sim.compute_pairwise(zero_first=True)
+        sim.compute_bonded()
+        out.append(gather(sim, lip, prot))
+    (f0, t0), (f1, t1) = out
+    assert eps_f(f1, f0) < tol and eps_f(t1, t0) < tol
+
+
+def _min_image_diff(a, b, box):
+    lo, hi, per = box
+    d = a - b
+    for k in range(3):
+        if per[k]:
+            L = hi[k] - lo[k]
+            d[:, k] -= L * np.round(d[:, k] / L)
+    return d
+
+
+def test_periodic_langevin_run_tracks_oracle(pbox):
+    """Small skin: rebuilds (and wraps) inside the batched run."""
+    s = periodic_patch(14, 12, seed=3)
+    seed = 9
+    sim, lip, prot = make_psim(s, orbc.FP64, seed=seed, skin=0.03)
+    pbox.set_box(s["box"])
+    st = {k: s[k].copy() for k in ("x", "v", "n", "o")}
+    st.update(type=s["type"].astype(np.int32), id=s["id"].astype(np.uint32))
+    nsteps = 40
+    sim.run(nsteps)
+    for step in range(nsteps):
+        f, t, _, _ = pbox.forces(st["x"], st["n"], st["type"], bonds=s["bonds"])
+        st["f"], st["t"] = f, t
+        pbox.langevin(st, 0.22, 0.01, 0.01, seed, step)
+    got = gather(sim, lip, prot, "state")
+    assert np.abs(_min_image_diff(got["x"], st["x"], s["box"])).max() < 1e-8
+    assert np.abs(got["n"] - st["n"]).max() < 1e-8
+    assert sim.observables()["rebuilds"] > 3
+
+
+def test_periodic_nose_hoover_run_tracks_oracle(pbox):
+    """orbc_run with the Nose-Hoover thermostat (the planar-patch workload's
+    integrator) against the oracle stepping the fused NH halves, src/openrbc.cpp:176-243."""
+    s = periodic_patch(14, 12, seed=5)
+    n = len(s["type"])
+    Q, kT, dt, zeta = 0.01 * n, 0.22, 0.01, 0.0
+    sim, lip, prot = make_psim(s, orbc.FP64, thermostat=orbc.RUN_NOSE_HOOVER, Q=Q, kBT=kT, skin=0.05)
+    pbox.set_box(s["box"])
+    st = {k: s[k].copy() for k in ("x", "v", "n", "o")}
+    st.update(type=s["type"].astype(np.int32))
+    big = (-1e9, -1e9, -1e9), (1e9, 1e9, 1e9)           # periodic axes have no walls; z walls far away
+    nsteps = 30
+    sim.run(nsteps)
+
+    def forces():
+        f, t, _, _ = pbox.forces(st["x"], st["n"], st["type"], bonds=s["bonds"])
+        st["f"], st["t"] = f, t
+
+    forces()
+    ke = pbox.nh_second(st, dt, zeta)
+    zeta = pbox.nh_zeta(zeta, dt, Q, ke, n, kT)
+    for _ in range(nsteps):
+        ke = pbox.nh_first(st, dt, zeta, *big)
+        zeta = pbox.nh_zeta(zeta, dt, Q, ke, n, kT)
+        forces()
+        ke = pbox.nh_second(st, dt, zeta)
+        zeta = pbox.nh_zeta(zeta, dt, Q, ke, n, kT)
+    got = gather(sim, lip, prot, "state")
+    assert np.abs(_min_image_diff(got["x"], st["x"], s["box"])).max() < 1e-8
+    assert np.abs(got["v"] - st["v"]).max() < 1e-8
+    assert abs(sim.observables()["zeta"] - zeta) < 1e-9 * max(1.0, abs(zeta))
+
+
+def test_periodic_box_rejects_voronoi():
+    s = periodic_patch(10, 8, seed=1)
+    sim, _, _ = make_psim(s, orbc.FP64)
+    with pytest.raises(orbc.OrbcError):
+        sim.voronoi_init(0, 2)
+
+
+# ---- batched ensembles (one store, one launch per step, per-member kBT) -----------------
+def _ensemble_sim(e, members, precision, seed=13, **kw):
+    sel = np.isin(e["group"], members)
+    sim = orbc.Simulation(precision=precision, seed=seed, **kw)
+    x, nrm, ids, g = e["lipid_x"][sel], e["lipid_n"][sel], e["lipid_id"][sel], e["group"][sel]
+    # Maxwell velocities drawn for the whole ensemble, so a member alone gets the same ones
+    rng = np.random.default_rng(4)
+    v = (rng.normal(0, 1, e["lipid_x"].shape) * np.sqrt(e["kT"][e["group"]])[:, None])[sel]
+    sim.set_state(orbc.LIPID, x, nrm, ids, v=v)
+    sim.set_state(orbc.PROTEIN, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.uint32), type=np.zeros(0, np.int32))
+    return sim, sel, g, v
+
+
+@pytest.mark.parametrize("precision,tol", [(orbc.FP64, 1e-9), (orbc.MIXED, 1e-5)])
+def test_ensemble_equals_separate_runs(precision, tol):
+    """Three vesicles with different kBT in one store evolve exactly as each
+    vesicle alone at its own kBT (noise keyed by particle id and step)."""
+    e = orbc.vesicle_ensemble(n_members=3, n_min=900, n_max=2500, gap=10.0, seed=2)
+    nsteps = 25
+    ens, sel, g, v = _ensemble_sim(e, [0, 1, 2], precision)
+    remap = {m: k for k, m in enumerate([0, 1, 2])}
+    ens.set_groups(orbc.LIPID, [remap[m] for m in g], e["kT"])
+    ens.run(nsteps)
+    xe = ens.get_state(orbc.LIPID)["x"]
+    ids_e = e["lipid_id"][sel]
+    # no pair between members
+    p = ens.pairs()
+    gid = dict(zip(ids_e.tolist(), g.tolist()))
+    assert all(gid[a] == gid[b] for a, b in p.tolist())
+    for m in range(3):
+        one, sel1, _, _ = _ensemble_sim(e, [m], precision, kBT=float(e["kT"][m]))
+        one.run(nsteps)
+        x1 = one.get_state(orbc.LIPID)["x"]
+        xe_m = xe[g == m]
+        assert np.abs(xe_m - x1).max() < tol * max(1.0, np.abs(x1).max()), m
+
+
+def test_ensemble_rejects_nose_hoover():
+    e = orbc.vesicle_ensemble(n_members=2, n_min=500, n_max=600, seed=1)
+    sim, sel, g, _ = _ensemble_sim(e, [0, 1], orbc.FP64, thermostat=orbc.RUN_NOSE_HOOVER)
+    sim.set_groups(orbc.LIPID, g, e["kT"])
+    with pytest.raises(orbc.OrbcError):
+        sim.run(2)
+
+
+def test_ensemble_full_size_properties():
+    """The batched-ensemble workload at its full shape (64 vesicles, 5k-300k
+    lipids each, ~4.5M particles, mixed precision): after minimisation and
+    500 Langevin steps no pair joins two members, every state is finite, and
+    each member's temperature stays within 25 % of its own kBT."""
+    e = orbc.vesicle_ensemble(n_members=64, seed=0)
+    g = e["group"]
+    sim, sel, _, _ = _ensemble_sim(e, list(range(64)), orbc.MIXED, skin=0.6)
+    sim.set_groups(orbc.LIPID, g, e["kT"])
+    sim.minimize(100)
+    rng = np.random.default_rng(5)
+    st = sim.get_state(orbc.LIPID)
+    v = rng.normal(0, 1, st["x"].shape) * np.sqrt(e["kT"][g])[:, None]
+    sim.set_state(orbc.LIPID, st["x"], st["n"], e["lipid_id"], v=v)
+    sim.run(500)
+    st = sim.get_state(orbc.LIPID)
+    assert np.isfinite(st["x"]).all() and np.isfinite(st["v"]).all()
+    T = np.bincount(g, weights=np.sum(st["v"] ** 2, 1), minlength=64) / (3 * np.bincount(g, minlength=64))
+    assert np.all(np.abs(T / e["kT"] - 1) < 0.25), (T / e["kT"])
+    p = sim.pairs()
+    gid = np.zeros(len(g) + 2, np.int32)
+    gid[e["lipid_id"]] = g
+    assert np.array_equal(gid[p[:, 0]], gid[p[:, 1]])
