@@ -185,6 +185,45 @@ def host_threads() -> int:
     return max(1, min(n, 16))
 
 
+def side_workload(args):
+    """The batched-ensemble and planar-membrane workloads: one JSON line with
+    the same metric on that workload (mixed precision)."""
+    import openrbc_amd as orbc
+    t0 = time.perf_counter()
+    if args.workload == "ensemble":
+        e = orbc.vesicle_ensemble(n_members=64, seed=0)
+        sim = orbc.Simulation(precision=orbc.MIXED, seed=7, skin=args.skin)
+        rng = np.random.default_rng(7)
+        v = rng.normal(0, 1, e["lipid_x"].shape) * np.sqrt(e["kT"][e["group"]])[:, None]
+        sim.set_state(orbc.LIPID, e["lipid_x"], e["lipid_n"], e["lipid_id"], v=v)
+        sim.set_state(orbc.PROTEIN, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.uint32),
+                      type=np.zeros(0, np.int32))
+        sim.set_groups(orbc.LIPID, e["group"], e["kT"])
+        n = len(e["lipid_x"])
+        desc = (f"batched ensemble: 64 lipid vesicles, {n} particles (5k-300k each, log-uniform), member kBT "
+                f"0.22 x U(0.9, 1.1), Langevin, one store and one fused launch per step")
+        steps = args.steps if args.steps != 1000 else 500
+    else:
+        sysd = orbc.planar_membrane(seed=1)
+        lo, hi, per = sysd["box"]
+        sim = orbc.Simulation(precision=orbc.MIXED, thermostat=orbc.RUN_NOSE_HOOVER, skin=args.skin,
+                              box_lo=list(lo), box_hi=list(hi), periodic=[int(p) for p in per])
+        orbc.system_to_simulation(sysd, sim)
+        n = len(sysd["lipid_x"]) + len(sysd["protein_x"])
+        desc = (f"periodic planar membrane: 4096 x 4096 lattice, {len(sysd['lipid_x'])} lipids + "
+                f"{len(sysd['protein_x'])} proteins, {len(sysd['bond_type'])} bonds, Nose-Hoover")
+        steps = args.steps if args.steps != 1000 else 200
+    print(f"# {desc} (set-up {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    sim.minimize(args.minimize if args.minimize != 1000 else 100)
+    wall, tm, obs, rebuilds, obs0 = timed_run(sim, steps, min(args.warmup, 20))
+    line = {"metric": METRIC, "value": n * steps / wall, "unit": "particle-timesteps/s", "n_gpus": 1,
+            "steps": steps, "warmup": min(args.warmup, 20), "ms_per_step": 1e3 * wall / steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 forces / fp64 state",
+            "data": "synthetic", "config": {"workload": desc, "n_particles": n, "rebuilds": rebuilds,
+                                            "temperature": obs["temperature"]}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +239,10 @@ def main():
     ap.add_argument("--volume-strength", type=float, default=0.05,
                     help="constrain_volume strength (0: no volume constraint in the loop)")
     ap.add_argument("--lloyd", type=int, default=64, help="Lloyd iterations of the Voronoi init")
+    ap.add_argument("--workload", choices=["rbc", "ensemble", "planar"], default="rbc",
+                    help="rbc: the full-size RBC, mixed and FP64 (the headline line); ensemble: 64 vesicles "
+                         "of uneven size with their own kBT in one store (mixed, Langevin); planar: the "
+                         "4096 x 4096 periodic membrane with its spectrin network (mixed, Nose-Hoover)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run 3 plain device-to-device copies of MiB each (the reference kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -218,6 +261,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "rbc":
+        return side_workload(args)
 
     import openrbc_amd as orbc
     t_build = time.perf_counter()

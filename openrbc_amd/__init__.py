@@ -406,11 +406,11 @@ def compute_temperature(sim: Simulation) -> float:
     return sim.compute_temperature()
 
 
-from .systems import (evans_fung, icosphere, init_rbc, load_example_large, planar_patch,  # noqa: E402
+from .systems import (evans_fung, icosphere, init_rbc, load_example_large, planar_membrane, planar_patch,  # noqa: E402
                       random_sphere, system_to_simulation, vesicle_ensemble)
 
 __all__ = [
     "Config", "Container", "Simulation", "OrbcError", "default_config", "load_library",
     "compute_pairwise_fused", "compute_bonded", "integrate", "constrain_volume", "compute_temperature",
-    "evans_fung", "icosphere", "init_rbc", "load_example_large", "planar_patch", "random_sphere", "system_to_simulation", "vesicle_ensemble",
+    "evans_fung", "icosphere", "init_rbc", "load_example_large", "planar_membrane", "planar_patch", "random_sphere", "system_to_simulation", "vesicle_ensemble",
 ]

@@ -56,11 +56,14 @@ __device__ inline double r2_rn(double x, double y, double z) { return __fma_rn(z
 // through the same code path (lipid-lipid and lipid-protein are both the
 // anisotropic form with different parameters), so a wave does not diverge on
 // the pair type except for the rare LJ / protein-protein terms.
+// The FP64 copies cut_d / rep_d / att_d serve the radial scalars of the
+// mixed anisotropic term (see aniso_term).
 template <typename R>
 struct FFS {
     R cut2p[36], cutp[36], rep[36], att[36], alpha[36], cut2lj[36], lj1[36], lj2[36];
     int aniso[36];
     R K[NBT], r0[NBT];
+    double cut_d[36], rep_d[36], att_d[36];
 };
 
 template <typename R>
@@ -70,6 +73,7 @@ __device__ inline void load_ffs(FFS<R> *s) {
         s->cut2p[k] = t.cut2_poly[k]; s->cutp[k] = t.cut_poly[k]; s->rep[k] = t.rep[k]; s->att[k] = t.att[k];
         s->alpha[k] = t.alpha[k]; s->cut2lj[k] = t.cut2_lj[k]; s->lj1[k] = t.lj1[k]; s->lj2[k] = t.lj2[k];
         s->aniso[k] = t.aniso[k];
+        s->cut_d[k] = c_ffd.cut_poly[k]; s->rep_d[k] = c_ffd.rep[k]; s->att_d[k] = c_ffd.att[k];
     }
     if (threadIdx.x < NBT) { s->K[threadIdx.x] = t.bond_K[threadIdx.x]; s->r0[threadIdx.x] = t.bond_r0[threadIdx.x]; }
     __syncthreads();
@@ -91,50 +95,121 @@ __device__ inline void radius(double r2, double &r, double &rinv) {
 // Anisotropic polynomial term of one directed pair inside its cutoff
 // (src/pairwise_kernel.h:37-62): force on i, director force on i, energy and
 // virial halves.
-template <typename R, bool ENERGY>
-__device__ __forceinline__ void aniso_term(R r2, R dx, R dy, R dz, R cut, R rep, R att, R alpha, R nix, R niy, R niz,
-                                           R njx, R njy, R njz, Acc<R> &acc) {
-    R r, rinv;
-    radius(r2, r, rinv);
-    const R ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
-    const R s = cut - r;
-    const R s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
-    const R ninj = nix * njx + niy * njy + niz * njz;
-    const R nie = nix * ex + niy * ey + niz * ez;
-    const R nje = njx * ex + njy * ey + njz * ez;
-    const R a = ninj - nie * nje;
-    const R A = R(1) + alpha * (a - R(1));
-    const R pix = nix - nie * ex, piy = niy - nie * ey, piz = niz - nie * ez;
-    const R pjx = njx - nje * ex, pjy = njy - nje * ey, pjz = njz - nje * ez;
-    const R ua = att * s4;
-    const R aua = alpha * ua;
-    const R auar = aua * rinv;
-    const R fra = s3 * (R(8) * rep * s4 + R(4) * A * att);
-    const R fx = fra * ex + auar * (nje * pix + nie * pjx);
-    const R fy = fra * ey + auar * (nje * piy + nie * pjy);
-    const R fz = fra * ez + auar * (nje * piz + nie * pjz);
-    acc.tx -= aua * pjx;
-    acc.ty -= aua * pjy;
-    acc.tz -= aua * pjz;
-    acc.fx += fx; acc.fy += fy; acc.fz += fz;
-    if (ENERGY) {
-        acc.e += R(0.5) * s4 * (rep * s4 + A * att);
-        acc.w += R(0.5) * (dx * fx + dy * fy + dz * fz);
+//
+// Mixed-path arithmetic (ORBC_PAIR_F64): the radial factor carries
+// (cut - r)^7 and A = 1 + alpha (a - 1) with alpha up to 5, so FP32 rounding
+// of r, cut or a is amplified into relative errors near 1e-6 on the stiff
+// lipid-protein terms.
+//   0: all FP32;
+//   1: r, s = cut - r, A and the radial scalars in FP64, vectors in FP32;
+//   2: the whole term in FP64 from the exact separation of the FP32 mirrors
+//      (X = (r_i - r_j) + h dA, formed in FP64 without rounding) and the
+//      FP32 directors, summed in FP64 (FP64 and FP32 FMA issue at the same
+//      rate on this GPU).  Rounding dx to FP32 would be the largest error.
+// The selection test stays FP32 in every mode.
+#ifndef ORBC_PAIR_F64
+#define ORBC_PAIR_F64 2
+#endif
+template <typename R, bool ENERGY, typename AR>
+__device__ __forceinline__ void aniso_term(const FFS<R> &t, int k, R r2, R dx, R dy, R dz, double X, double Y,
+                                           double Z, R nix, R niy, R niz, R njx, R njy, R njz, Acc<AR> &acc) {
+    if constexpr (std::is_same<R, float>::value && ORBC_PAIR_F64 >= 2) {
+        const double d2 = __fma_rn(Z, Z, __fma_rn(Y, Y, X * X));
+        double y = (double)rsq(r2);                               // 1/r to FP32 accuracy
+        y = y * __fma_rn(-0.5 * d2, y * y, 1.5);                  // one Newton step: FP64 accuracy
+        const double rd = d2 * y;
+        const double ex = X * y, ey = Y * y, ez = Z * y;
+        const double ax = nix, ay = niy, az = niz, bx = njx, by = njy, bz = njz;
+        const double ninj = __fma_rn(az, bz, __fma_rn(ay, by, ax * bx));
+        const double nie = __fma_rn(az, ez, __fma_rn(ay, ey, ax * ex));
+        const double nje = __fma_rn(bz, ez, __fma_rn(by, ey, bx * ex));
+        const double a = __fma_rn(-nie, nje, ninj);
+        const double pix = __fma_rn(-nie, ex, ax), piy = __fma_rn(-nie, ey, ay), piz = __fma_rn(-nie, ez, az);
+        const double pjx = __fma_rn(-nje, ex, bx), pjy = __fma_rn(-nje, ey, by), pjz = __fma_rn(-nje, ez, bz);
+        const double s = t.cut_d[k] - rd;
+        const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+        const double alpha = t.alpha[k];
+        const double A = __fma_rn(alpha, a - 1.0, 1.0);
+        const double rep = t.rep_d[k], att = t.att_d[k];
+        const double aua = alpha * (att * s4);
+        const double auar = aua * y;
+        const double fra = s3 * __fma_rn(4.0 * A, att, 8.0 * rep * s4);
+        const double fx = __fma_rn(fra, ex, auar * __fma_rn(nje, pix, nie * pjx));
+        const double fy = __fma_rn(fra, ey, auar * __fma_rn(nje, piy, nie * pjy));
+        const double fz = __fma_rn(fra, ez, auar * __fma_rn(nje, piz, nie * pjz));
+        acc.tx -= (AR)(aua * pjx);
+        acc.ty -= (AR)(aua * pjy);
+        acc.tz -= (AR)(aua * pjz);
+        acc.fx += (AR)fx; acc.fy += (AR)fy; acc.fz += (AR)fz;
+        if (ENERGY) {
+            acc.e += (AR)(0.5 * s4 * __fma_rn(A, att, rep * s4));
+            acc.w += (AR)(0.5 * __fma_rn(Z, fz, __fma_rn(Y, fy, X * fx)));
+        }
+        return;
+    } else {
+        R r, rinv;
+        radius(r2, r, rinv);
+        const R alpha = t.alpha[k];
+        const R ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+        const R ninj = nix * njx + niy * njy + niz * njz;
+        const R nie = nix * ex + niy * ey + niz * ez;
+        const R nje = njx * ex + njy * ey + njz * ez;
+        const R a = ninj - nie * nje;
+        const R pix = nix - nie * ex, piy = niy - nie * ey, piz = niz - nie * ez;
+        const R pjx = njx - nje * ex, pjy = njy - nje * ey, pjz = njz - nje * ez;
+        R fra, aua, auar, en = R(0);
+        if constexpr (std::is_same<R, float>::value && ORBC_PAIR_F64 == 1) {
+            const double X = dx, Y = dy, Z = dz;
+            const double d2 = __fma_rn(Z, Z, __fma_rn(Y, Y, X * X));
+            double y = (double)rinv;
+            y = y * __fma_rn(-0.5 * d2, y * y, 1.5);
+            const double rd = d2 * y;
+            const double s = t.cut_d[k] - rd;
+            const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+            const double A = __fma_rn((double)alpha, (double)a - 1.0, 1.0);
+            const double rep = t.rep_d[k], att = t.att_d[k];
+            const double aua_d = (double)alpha * (att * s4);
+            fra = (R)(s3 * __fma_rn(4.0 * A, att, 8.0 * rep * s4));
+            aua = (R)aua_d;
+            auar = (R)(aua_d * y);
+            if (ENERGY) en = (R)(0.5 * s4 * __fma_rn(A, att, rep * s4));
+        } else {
+            const R s = t.cutp[k] - r;
+            const R s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+            const R A = R(1) + alpha * (a - R(1));
+            const R rep = t.rep[k], att = t.att[k];
+            aua = alpha * (att * s4);
+            auar = aua * rinv;
+            fra = s3 * (R(8) * rep * s4 + R(4) * A * att);
+            if (ENERGY) en = R(0.5) * s4 * (rep * s4 + A * att);
+        }
+        const R fx = fra * ex + auar * (nje * pix + nie * pjx);
+        const R fy = fra * ey + auar * (nje * piy + nie * pjy);
+        const R fz = fra * ez + auar * (nje * piz + nie * pjz);
+        acc.tx -= (AR)(aua * pjx);
+        acc.ty -= (AR)(aua * pjy);
+        acc.tz -= (AR)(aua * pjz);
+        acc.fx += (AR)fx; acc.fy += (AR)fy; acc.fz += (AR)fz;
+        if (ENERGY) {
+            acc.e += (AR)en;
+            acc.w += (AR)(R(0.5) * (dx * fx + dy * fy + dz * fz));
+        }
     }
 }
 
 // One directed pair (owner i, neighbour j, d = x_i - x_j).  Returns true if
 // the pair interacts (same selection test as the oracle, bit for bit).
-template <typename R, bool ENERGY>
-__device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R dy, R dz, R r2, R nix, R niy,
-                                        R niz, R njx, R njy, R njz, Acc<R> &acc) {
+// X, Y, Z: the separation in FP64 (the mixed path's exact mirror separation;
+// equal to dx, dy, dz in the FP64 path).
+template <typename R, bool ENERGY, typename AR>
+__device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R dy, R dz, double X, double Y,
+                                        double Z, R r2, R nix, R niy, R niz, R njx, R njy, R njz, Acc<AR> &acc) {
     if (!(r2 > R(1e-5))) return false;
     const int k = ti * NT + tj;
     R fx, fy, fz;
     if (r2 < t.cut2p[k]) {
         if (t.aniso[k]) {
-            aniso_term<R, ENERGY>(r2, dx, dy, dz, t.cutp[k], t.rep[k], t.att[k], t.alpha[k], nix, niy, niz, njx, njy,
-                                  njz, acc);
+            aniso_term<R, ENERGY>(t, k, r2, dx, dy, dz, X, Y, Z, nix, niy, niz, njx, njy, njz, acc);
             return true;
         }
         // src/pairwise_kernel_fused.h:79-97
@@ -146,7 +221,7 @@ __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R
         const R rep = t.rep[k];
         const R fr = R(8) * rep * s3 * s4;
         fx = fr * ex; fy = fr * ey; fz = fr * ez;
-        if (ENERGY) acc.e += R(0.5) * rep * s4 * s4;
+        if (ENERGY) acc.e += (AR)(R(0.5) * rep * s4 * s4);
     } else {
         if (!(r2 < t.cut2lj[k])) return false;
         // src/pairwise_kernel_fused.h:63-77 (WCA, unshifted)
@@ -154,10 +229,10 @@ __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R
         const R r6inv = r2inv * r2inv * r2inv;
         const R fpair = r6inv * (t.lj1[k] * r6inv - t.lj2[k]) * r2inv;
         fx = dx * fpair; fy = dy * fpair; fz = dz * fpair;
-        if (ENERGY) acc.e += R(0.5) * r6inv * (t.lj1[k] * (R(1) / R(12)) * r6inv - t.lj2[k] * (R(1) / R(6)));
+        if (ENERGY) acc.e += (AR)(R(0.5) * r6inv * (t.lj1[k] * (R(1) / R(12)) * r6inv - t.lj2[k] * (R(1) / R(6))));
     }
-    acc.fx += fx; acc.fy += fy; acc.fz += fz;
-    if (ENERGY) acc.w += R(0.5) * (dx * fx + dy * fy + dz * fz);
+    acc.fx += (AR)fx; acc.fy += (AR)fy; acc.fz += (AR)fz;
+    if (ENERGY) acc.w += (AR)(R(0.5) * (dx * fx + dy * fy + dz * fz));
     return true;
 }
 
@@ -215,8 +290,9 @@ constexpr int UNROLL = ORBC_STEP_UNROLL;
 // indices together with this group's gathers; a group is padded with the
 // particle itself (r2 = 0, no contribution).
 
-// Mixed: FP32 pair terms from the FP32 mirrors, summed in FP32 over each group
-// and in FP64 across groups.
+// Mixed: pair terms from the FP32 mirrors, summed in FP64 (ORBC_PAIR_F64 >= 2)
+// or in FP32 over each group and in FP64 across groups.
+using PairAcc = std::conditional<(ORBC_PAIR_F64 >= 2), double, float>::type;
 template <bool ENERGY>
 __device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long long stride, const PQ *pq,
                                             const uint32_t *nb, const Anchor &ag, const AnchorI &ai, float4 xi,
@@ -235,18 +311,28 @@ __device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long lo
         float4 xj[UNROLL], nj[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) { xj[u] = pq[jj[u]].x; nj[u] = pq[jj[u]].n; }
-        Acc<float> acc;
+        Acc<float> acc32;
+        // FP64 terms go straight into the FP64 sums
+        Acc<PairAcc> &acc = *[&]() {
+            if constexpr (ORBC_PAIR_F64 >= 2) return &out; else return &acc32;
+        }();
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const AnchorI aj = unpack(__float_as_uint(xj[u].w), ag);
-            const float dx = add_rn(sub_rn(xi.x, xj[u].x), anchor_dx(ag, 0, ai.x - aj.x));
-            const float dy = add_rn(sub_rn(xi.y, xj[u].y), anchor_dx(ag, 1, ai.y - aj.y));
-            const float dz = add_rn(sub_rn(xi.z, xj[u].z), anchor_dx(ag, 2, ai.z - aj.z));
+            const float dx = anchor_add(ag, 0, sub_rn(xi.x, xj[u].x), ai.x - aj.x);
+            const float dy = anchor_add(ag, 1, sub_rn(xi.y, xj[u].y), ai.y - aj.y);
+            const float dz = anchor_add(ag, 2, sub_rn(xi.z, xj[u].z), ai.z - aj.z);
             const float r2 = r2_rn(dx, dy, dz);
-            pair_ij<float, ENERGY>(t, ti, __float_as_int(nj[u].w), dx, dy, dz, r2, ni.x, ni.y, ni.z, nj[u].x,
-                                   nj[u].y, nj[u].z, acc);
+            double X = 0.0, Y = 0.0, Z = 0.0;
+            if constexpr (ORBC_PAIR_F64 >= 2) {
+                X = anchor_add_exact(ag, 0, xi.x, xj[u].x, ai.x - aj.x);
+                Y = anchor_add_exact(ag, 1, xi.y, xj[u].y, ai.y - aj.y);
+                Z = anchor_add_exact(ag, 2, xi.z, xj[u].z, ai.z - aj.z);
+            }
+            pair_ij<float, ENERGY>(t, ti, __float_as_int(nj[u].w), dx, dy, dz, X, Y, Z, r2, ni.x, ni.y, ni.z,
+                                   nj[u].x, nj[u].y, nj[u].z, acc);
         }
-        add_to(out, acc);
+        if constexpr (ORBC_PAIR_F64 < 2) add_to(out, acc32);
     }
 }
 
@@ -269,9 +355,9 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
             const uint32_t j = e & BOND_INDEX_MASK;
             const float4 xj = pq[j].x;
             const AnchorI aj = unpack(__float_as_uint(xj.w), ag);
-            const float dx = (xj.x - xi.x) + anchor_dx(ag, 0, aj.x - ai.x);
-            const float dy = (xj.y - xi.y) + anchor_dx(ag, 1, aj.y - ai.y);
-            const float dz = (xj.z - xi.z) + anchor_dx(ag, 2, aj.z - ai.z);
+            const float dx = anchor_add(ag, 0, sub_rn(xj.x, xi.x), aj.x - ai.x);
+            const float dy = anchor_add(ag, 1, sub_rn(xj.y, xi.y), aj.y - ai.y);
+            const float dz = anchor_add(ag, 2, sub_rn(xj.z, xi.z), aj.z - ai.z);
             bond_ij<float, ENERGY>(t, (int)(e >> 30), dx, dy, dz, acc);
         }
         add_to(out, acc);
@@ -309,7 +395,8 @@ __device__ __forceinline__ void range_fp64(long long i, int kb, int ke, long lon
             double dz = sub_rn(zi, xj[u][2]);
             if (periodic) min_image(ag, dx, dy, dz);
             const double r2 = r2_rn(dx, dy, dz);
-            pair_ij<double, ENERGY>(t, ti, tj[u], dx, dy, dz, r2, nix, niy, niz, nj[u][0], nj[u][1], nj[u][2], acc);
+            pair_ij<double, ENERGY>(t, ti, tj[u], dx, dy, dz, dx, dy, dz, r2, nix, niy, niz, nj[u][0], nj[u][1],
+                                    nj[u][2], acc);
         }
     }
 }
@@ -377,9 +464,9 @@ __global__ __launch_bounds__(BLOCK) void k_pairs_export(ForceArgs a, unsigned *i
         if (MIXED) {
             const float4 xi = a.pq[i].x, xj = a.pq[j].x;
             const AnchorI ai = unpack(__float_as_uint(xi.w), a.anc_grid), aj = unpack(__float_as_uint(xj.w), a.anc_grid);
-            const float dx = add_rn(sub_rn(xi.x, xj.x), anchor_dx(a.anc_grid, 0, ai.x - aj.x));
-            const float dy = add_rn(sub_rn(xi.y, xj.y), anchor_dx(a.anc_grid, 1, ai.y - aj.y));
-            const float dz = add_rn(sub_rn(xi.z, xj.z), anchor_dx(a.anc_grid, 2, ai.z - aj.z));
+            const float dx = anchor_add(a.anc_grid, 0, sub_rn(xi.x, xj.x), ai.x - aj.x);
+            const float dy = anchor_add(a.anc_grid, 1, sub_rn(xi.y, xj.y), ai.y - aj.y);
+            const float dz = anchor_add(a.anc_grid, 2, sub_rn(xi.z, xj.z), ai.z - aj.z);
             const float r2 = r2_rn(dx, dy, dz);
             const int kk = __float_as_int(a.pq[i].n.w) * NT + __float_as_int(a.pq[j].n.w);
             hit = r2 > 1e-5f && (r2 < c_fff.cut2_poly[kk] || r2 < c_fff.cut2_lj[kk]);

@@ -142,6 +142,104 @@ __device__ int nearest_warm(const VoroArgs &a, float p0, float p1, float p2, int
     return bi;
 }
 
+// Warm partition of the particle store, one wavefront per 64 consecutive
+// (Morton-ordered, hence nearby) particles.  Each lane seeds its search ball
+// from its previous cell (radius |p - c_k0| plus the centroid drift, as in
+// nearest_warm); the union of the 64 balls is a box, the centroids binned in
+// the grid cells overlapping it are staged in LDS, and every lane scans them
+// all by broadcast reads.  Same answer as nearest_warm (lowest index on equal
+// distance); a wave whose box holds too many cells or centroids falls back to
+// nearest_warm per lane.
+constexpr int NW_CAP = 256;
+__global__ __launch_bounds__(BLOCK) void k_nearest_wave(VoroArgs a, const double *qd, long long nq, int32_t *out) {
+    if (gate_closed(a.gate)) return;
+    __shared__ float4 s_c[BLOCK / 64][NW_CAP];
+    __shared__ uint32_t s_pref[BLOCK / 64][65];
+    __shared__ uint32_t s_st[BLOCK / 64][64];
+    const int w = threadIdx.x >> 6, lane = __lane_id();
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const bool act = i < nq;
+    const Grid &g = a.cg;
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, best = INFINITY, r = 0.f;
+    int bi = -1, k0 = -1;
+    if (act) {
+        p0 = (float)qd[3 * i]; p1 = (float)qd[3 * i + 1]; p2 = (float)qd[3 * i + 2];
+        k0 = out[i];
+        if (k0 >= 0 && k0 < a.ncell) {
+            best = normsq_unfused(p0 - a.centroids[3 * k0], p1 - a.centroids[3 * k0 + 1], p2 - a.centroids[3 * k0 + 2]);
+            bi = k0;
+            r = sqrtf(best) * 1.0001f + 1e-4f + centroid_drift(a);
+        }
+    }
+    // wave box of the search balls (lanes without a guess use the full search below)
+    const bool guessed = act && bi >= 0;
+    const bool all_guessed = __ballot(guessed) == __ballot(act);
+    float lo0 = guessed ? p0 - r : INFINITY, lo1 = guessed ? p1 - r : INFINITY, lo2 = guessed ? p2 - r : INFINITY;
+    float hi0 = guessed ? p0 + r : -INFINITY, hi1 = guessed ? p1 + r : -INFINITY, hi2 = guessed ? p2 + r : -INFINITY;
+    for (int off = 32; off > 0; off >>= 1) {
+        lo0 = fminf(lo0, __shfl_xor(lo0, off, 64)); lo1 = fminf(lo1, __shfl_xor(lo1, off, 64));
+        lo2 = fminf(lo2, __shfl_xor(lo2, off, 64)); hi0 = fmaxf(hi0, __shfl_xor(hi0, off, 64));
+        hi1 = fmaxf(hi1, __shfl_xor(hi1, off, 64)); hi2 = fmaxf(hi2, __shfl_xor(hi2, off, 64));
+    }
+    bool wave_ok = all_guessed && __ballot(act) != 0ull;
+    int x0 = 0, x1 = -1, y0 = 0, y1 = -1, z0 = 0, z1 = -1, nx = 0, ny = 0, ncl = 0;
+    if (wave_ok) {
+        x0 = gcoord(lo0, g, 0); x1 = gcoord(hi0, g, 0);
+        y0 = gcoord(lo1, g, 1); y1 = gcoord(hi1, g, 1);
+        z0 = gcoord(lo2, g, 2); z1 = gcoord(hi2, g, 2);
+        nx = x1 - x0 + 1; ny = y1 - y0 + 1;
+        ncl = nx * ny * (z1 - z0 + 1);
+        wave_ok = ncl <= 64;
+    }
+    uint32_t total = 0;
+    if (wave_ok) {
+        uint32_t st = 0, cnt = 0;
+        if (lane < ncl) {
+            const int cx = x0 + lane % nx, cy = y0 + (lane / nx) % ny, cz = z0 + lane / (nx * ny);
+            const long long cl = glin(cx, cy, cz, g);
+            st = a.cg_start[cl];
+            cnt = a.cg_end[cl] - st;
+        }
+        uint32_t inc = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        total = __shfl(inc, 63, 64);
+        s_pref[w][lane + 1] = inc;
+        s_st[w][lane] = st;
+        if (lane == 0) s_pref[w][0] = 0;
+        wave_ok = total <= (uint32_t)NW_CAP;
+    }
+    if (wave_ok) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t t = lane; t < total; t += 64) {
+            int lo = 0, hi = 64;                    // largest k with pref[k] <= t
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pref[w][mid] <= t) lo = mid; else hi = mid;
+            }
+            const int k = (int)a.cg_idx[s_st[w][lo] + (t - s_pref[w][lo])];
+            s_c[w][t] = make_float4(a.centroids[3 * k], a.centroids[3 * k + 1], a.centroids[3 * k + 2],
+                                    __int_as_float(k));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t t = 0; t < total; ++t) {
+            const float4 c = s_c[w][t];
+            const int k = __float_as_int(c.w);
+            const float d = normsq_unfused(p0 - c.x, p1 - c.y, p2 - c.z);
+            if (d < best || (d == best && k < bi)) { best = d; bi = k; }
+        }
+        if (act) out[i] = bi;
+    } else if (act) {
+        out[i] = (bi >= 0) ? nearest_warm(a, p0, p1, p2, k0) : nearest_one(a, p0, p1, p2);
+    }
+}
+
 // warm: out[] holds the previous affiliation and is used as the guess
 __global__ __launch_bounds__(BLOCK) void k_nearest(VoroArgs a, const float *q, long long nq, const double *qd,
                                                    int32_t *out, int warm) {
@@ -419,7 +517,11 @@ void launch_cgrid_ranges(const uint32_t *sorted_keys, long long m, uint32_t *sta
 
 void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out, bool warm,
                     hipStream_t s) {
-    if (nq > 0) hipLaunchKernelGGL(k_nearest, dim3(nblk(nq)), dim3(BLOCK), 0, s, a, q, nq, qd, out, (int)warm);
+    if (nq <= 0) return;
+    if (warm && qd)
+        hipLaunchKernelGGL(k_nearest_wave, dim3(nblk(nq)), dim3(BLOCK), 0, s, a, qd, nq, out);
+    else
+        hipLaunchKernelGGL(k_nearest, dim3(nblk(nq)), dim3(BLOCK), 0, s, a, q, nq, qd, out, (int)warm);
 }
 
 void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, int cap, int *counts, int *out,

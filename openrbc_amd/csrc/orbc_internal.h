@@ -68,6 +68,17 @@ __device__ inline int anchor_wrap(int dA, unsigned s) { return (int)((unsigned)d
 __device__ inline float anchor_dx(const Anchor &a, int axis, int dA) {
     return a.hf[axis] * (float)anchor_wrap(dA, a.wrap[axis]);
 }
+// r + (distance between the anchors) with one rounding: on open axes the
+// product is exact (power-of-two spacing), on periodic axes it is not
+__device__ inline float anchor_add(const Anchor &a, int axis, float r, int dA) {
+    return __fmaf_rn(a.hf[axis], (float)anchor_wrap(dA, a.wrap[axis]), r);
+}
+// the same separation in FP64 without rounding: the offsets differ exactly in
+// FP64, and h * dA is exact on open axes (on periodic axes it carries the
+// rounding of h = L / 2^bits only)
+__device__ inline double anchor_add_exact(const Anchor &a, int axis, float ri, float rj, int dA) {
+    return __fma_rn(a.hd[axis], (double)anchor_wrap(dA, a.wrap[axis]), (double)ri - (double)rj);
+}
 // FP64 position of anchor A along one axis
 __device__ inline double anchor_pos(const Anchor &a, int axis, long long A) {
     return a.lo[axis] + a.hd[axis] * (double)(A + a.A0[axis]);

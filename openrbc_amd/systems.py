@@ -11,6 +11,7 @@
 * ``random_sphere``: init_random_sphere (src/init_random.h:27-66), lipids only.
 * ``planar_patch``: hexagonal lipid lattice (periodic patch workloads).
 * ``vesicle_ensemble``: many lipid vesicles of uneven size (batched ensembles).
+* ``planar_membrane``: periodic planar lipid lattice with a spectrin network beneath.
 """
 from __future__ import annotations
 
@@ -217,6 +218,96 @@ def vesicle_ensemble(n_members: int = 64, n_min: int = 5000, n_max: int = 300000
     x = np.vstack(xs)
     return dict(lipid_x=x, lipid_n=np.vstack(ns), lipid_id=np.arange(1, len(x) + 1, dtype=np.uint32),
                 group=np.concatenate(gs), kT=kts, radius=radius, centre=np.array(centres))
+
+
+def planar_membrane(nx: int = 4096, ny: int = 4096, a: float = 1.0487, edge: float = 16.84, tilt: float = 0.02,
+                    jitter: float = 0.02, layer: float = 2.25, seed: int = 0):
+    """Large periodic planar membrane: an nx x ny hexagonal lipid lattice
+    (lattice constant a, directors +z tilted by a Gaussian of width `tilt`
+    radians, positions jittered by jitter * a) tiling a box periodic in x and
+    y, with a triangular spectrin network of edge ~`edge` tethered beneath it
+    by the composition of src/init_rbc.h:122-233: per network vertex a
+    glycophorin in the membrane and an actin `layer` below it (bond type 0);
+    per edge an immobile band-3 at the midpoint in the membrane and a chain of
+    19 spectrin beads between the two actins (types 3, 2 to the actins, 1 from
+    the middle bead to the band-3).  Membrane proteins take the place of the
+    nearest lattice lipid.  Periodic bonds are given by their nearest image.
+    Returns the system dict (lipid_*, protein_*, bond_*) and box = (lo, hi,
+    periodic)."""
+    assert ny % 2 == 0
+    rng = np.random.default_rng(seed)
+    Lx, Ly = nx * a, ny * a * np.sqrt(3) / 2
+    # network vertices on a triangular lattice that tiles the box
+    mx = max(3, int(round(Lx / edge)))
+    my = max(4, int(round(Ly / (edge * np.sqrt(3) / 2))) // 2 * 2)
+    ex, ey = Lx / mx, Ly / my
+    pp, qq = np.meshgrid(np.arange(mx), np.arange(my), indexing="ij")
+    pp, qq = pp.ravel(), qq.ravel()
+    vx = ex * (pp + 0.5 * (qq % 2))
+    vy = ey * qq
+    nv = len(vx)
+    vid = lambda p, q: (p % mx) * my + (q % my)
+    # three edges per vertex: right, and the two upward neighbours
+    up_l = np.where(qq % 2 == 0, pp - 1, pp)
+    up_r = np.where(qq % 2 == 0, pp, pp + 1)
+    e0 = np.concatenate([vid(pp, qq)] * 3)
+    e1 = np.concatenate([vid(pp + 1, qq), vid(up_l, qq + 1), vid(up_r, qq + 1)])
+    ne = len(e0)
+
+    def mi(d, L):
+        return d - L * np.round(d / L)
+
+    def wrap(v, L):
+        return v - L * np.floor(v / L)
+
+    gly = np.stack([vx, vy, np.zeros(nv)], 1)
+    act = gly - np.array([0.0, 0.0, layer])
+    d_e = np.stack([mi(vx[e1] - vx[e0], Lx), mi(vy[e1] - vy[e0], Ly), np.zeros(ne)], 1)
+    mid = gly[e0] + 0.5 * d_e
+    mid[:, 0] = wrap(mid[:, 0], Lx); mid[:, 1] = wrap(mid[:, 1], Ly)
+    # spectrin: 19 beads between the actins, 1.0 in from each end (src/init_rbc.h:201-214)
+    res = 19
+    beg = act[e0]
+    inc = d_e / (res + 1)
+    u = inc / np.linalg.norm(inc, axis=1, keepdims=True)
+    beg = beg + u
+    inc = (d_e - 2 * u) / (res + 1)
+    j = np.arange(1, res + 1)
+    spec = beg[:, None, :] + inc[:, None, :] * j[None, :, None] + 0.05 * rng.uniform(-1, 1, (ne, res, 3))
+    spec = spec.reshape(-1, 3)
+    spec[:, 0] = wrap(spec[:, 0], Lx); spec[:, 1] = wrap(spec[:, 1], Ly)
+    # proteins in tag order: glycophorin, actin, band-3, spectrin (src/init_rbc.h:122-233)
+    px = np.vstack([gly, act, mid, spec])
+    ptype = np.concatenate([np.full(nv, 3), np.full(nv, 4), np.full(ne, 2), np.full(ne * res, 5)]).astype(np.int32)
+    pn = np.zeros_like(px); pn[:, 2] = 1.0
+    np_ = len(px)
+    tag = np.arange(1, np_ + 1, dtype=np.uint32)
+    t_gly, t_act, t_b3 = tag[:nv], tag[nv:2 * nv], tag[2 * nv:2 * nv + ne]
+    t_sp = tag[2 * nv + ne:].reshape(ne, res)
+    bt = [np.zeros(nv, np.int32), np.full(ne * (res - 1), 3, np.int32), np.full(ne, 2, np.int32),
+          np.full(ne, 1, np.int32), np.full(ne, 2, np.int32)]
+    bi = [t_gly, t_sp[:, :-1].ravel(), t_act[e0], t_b3, t_act[e1]]
+    bj = [t_act, t_sp[:, 1:].ravel(), t_sp[:, 0], t_sp[:, res // 2], t_sp[:, -1]]
+    # lipid lattice minus the sites nearest to the membrane proteins
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    taken = np.zeros((nx, ny), bool)
+    for q in (gly, mid):
+        jq = np.round(q[:, 1] / (a * np.sqrt(3) / 2)).astype(np.int64) % ny
+        iq = np.round(q[:, 0] / a - 0.5 * (jq % 2)).astype(np.int64) % nx
+        taken[iq, jq] = True
+    keep = ~taken.ravel()
+    lx = np.stack([(a * (ii + 0.5 * (jj % 2))).ravel()[keep], (a * (np.sqrt(3) / 2) * jj).ravel()[keep]], 1)
+    nl = len(lx)
+    lx = np.hstack([lx, np.zeros((nl, 1))]) + rng.normal(0, jitter * a, (nl, 3)) * np.array([1.0, 1.0, 0.0])
+    ln = np.zeros((nl, 3)); ln[:, 2] = 1.0
+    ln += rng.normal(0, tilt, (nl, 3)) * np.array([1.0, 1.0, 0.0])
+    ln /= np.linalg.norm(ln, axis=1, keepdims=True)
+    lo = np.array([0.0, 0.0, -1000.0]); hi = np.array([Lx, Ly, 1000.0])
+    return dict(lipid_x=lx, lipid_n=ln, lipid_id=np.arange(np_ + 1, np_ + 1 + nl, dtype=np.uint32),
+                protein_x=px, protein_n=pn, protein_type=ptype, protein_id=tag,
+                bond_type=np.concatenate(bt), bond_i=np.concatenate(bi).astype(np.uint32),
+                bond_j=np.concatenate(bj).astype(np.uint32), box=(lo, hi, np.array([1, 1, 0], np.int32)),
+                n_vertices=nv, n_edges=ne)
 
 
 def planar_patch(nx: int, ny: int, a: float = 1.0487, tilt: float = 0.02, jitter: float = 0.02, seed: int = 0):

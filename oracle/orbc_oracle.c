@@ -343,8 +343,7 @@ int64_t orc_pairs_brute_mixed_anchor(int64_t n, const double *x, const double *x
                 int32_t dA = (int32_t)(A[3 * i + k] - A[3 * j + k]);
                 if (bits[k] > 0) dA = (int32_t)((uint32_t)dA << (32 - bits[k])) >> (32 - bits[k]);
                 const float a = r[3 * i + k] - r[3 * j + k];
-                const float s = (float)h[k] * (float)dA;
-                dd[k] = a + s;
+                dd[k] = fmaf((float)h[k], (float)dA, a);
             }
             const float r2 = fmaf(dd[2], dd[2], fmaf(dd[1], dd[1], dd[0] * dd[0]));
             const int k = type[i] * 6 + type[j];

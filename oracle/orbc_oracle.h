@@ -78,7 +78,7 @@ int64_t orc_pairs_cell_fp64(int64_t n, const double *x, const int32_t *type,
 /* Mixed selection with the general anchor scheme of the device, per axis d:
  * A = floor((xa - lo_d) / h_d) - A0_d, clamped to [0, 2^bits_d - 1] on
  * periodic axes, r = (float)(x - (lo_d + h_d (A + A0_d))), and
- * dx = fl(fl(r_i - r_j) + fl((float)h_d * (float)dA)) where on a periodic axis
+ * dx = fma((float)h_d, (float)dA, fl(r_i - r_j)) (one rounding) where on a periodic axis
  * dA = A_i - A_j sign-extended from bits_d (wrap to the nearest image).
  * Open axes: lo_d = 0, h_d a power of two, bits_d = 0 (no wrap). */
 int64_t orc_pairs_brute_mixed_anchor(int64_t n, const double *x, const double *xa, const int32_t *type,

@@ -80,7 +80,8 @@ def split(s):
 
 def periodic_patch(nx=16, ny=16, seed=0, spacing=1.0487, origin=(-3.0, 5.0), tilt=0.05, jitter=0.05):
     """Hexagonal lipid sheet that tiles a periodic box in x and y (ny even),
-    z open: glycophorin (type 3) and band-3 (type 2) replace sheet sites,
+    z open: glycophorin (type 3) and band-3 (type 2) replace sheet sites
+    (lipids nearer than 1.685 to them are removed),
     actin (type 4) sits 2.25 below each glycophorin with a type-0 bond.
     Positions are jittered and may fall outside the box (the device wraps
     them).  Returns the system dict plus box = (lo, hi, periodic)."""
@@ -94,10 +95,39 @@ def periodic_patch(nx=16, ny=16, seed=0, spacing=1.0487, origin=(-3.0, 5.0), til
     nrm = np.tile([0.0, 0.0, 1.0], (len(pts), 1)) + rng.normal(0, tilt, (len(pts), 3))
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
     types = np.zeros(len(pts), np.int32)
-    sites = rng.choice(len(pts), size=max(2, len(pts) // 30), replace=False)
-    gly, b3 = sites[: len(sites) // 2], sites[len(sites) // 2:]
+    nprot = max(2, len(pts) // 30)
+    Lb = np.array([nx * spacing, ny * spacing * np.sqrt(3) / 2])
+
+    def pdist(a, b):                      # in-plane minimum-image distances
+        d = a[:, None, :2] - b[None, :, :2]
+        d -= Lb * np.round(d / Lb)
+        return np.linalg.norm(d, axis=2)
+
+    def spread(cand, k, dmin):            # up to k sites of cand, pairwise farther than dmin
+        out = []
+        for c in rng.permutation(cand):
+            if len(out) == k:
+                break
+            if not out or pdist(pts[[c]], pts[out]).min() > dmin:
+                out.append(int(c))
+        return np.array(out, dtype=np.int64)
+
+    # membrane proteins spaced as on the reference's mesh (vertices and edge
+    # midpoints, src/init_rbc.h:122-172): no two within the protein-protein
+    # cutoff 2.6, band-3 more than 4 from a glycophorin (a band-3 next to the
+    # actin under a glycophorin would sit deep inside their sigma-3.4 WCA core)
+    gly = spread(np.arange(len(pts)), nprot // 2, 2.6)
+    far = np.where(pdist(pts, pts[gly]).min(axis=1) > 4.0)[0]
+    b3 = spread(far, nprot - len(gly), 2.6)
     types[gly] = 3
     types[b3] = 2
+    # lipids closer than reqlp = 1.685 to a protein are removed, as the
+    # reference's init does (src/init_rbc.h:243-282)
+    mem = np.concatenate([gly, b3])
+    keep = (types != 0) | (pdist(pts, pts[mem]).min(axis=1) >= 1.685)
+    idx = np.cumsum(keep) - 1
+    pts, nrm, types = pts[keep], nrm[keep], types[keep]
+    gly, b3 = idx[gly], idx[b3]
     n0 = len(pts)
     act = pts[gly] - np.array([0, 0, 2.25]) + rng.normal(0, 0.05, (len(gly), 3))
     pts = np.vstack([pts, act])
