@@ -114,6 +114,8 @@ typedef struct orbc_observables {
        anchor_bits is the packed width (anchor differences wrap by it on periodic axes) */
     double anchor_lo[3], anchor_spacing[3], anchor_origin[3];
     int anchor_bits[3];
+    long long n_lipid, n_protein;   /* species counts (lipids shrink with the stray-lipid cleanup) */
+    long long lipids_deleted;       /* lipids removed by the cleanup so far */
 } orbc_observables;
 
 void orbc_config_default(orbc_config *cfg);
@@ -184,6 +186,15 @@ int orbc_voronoi_set_centroids(orbc_ctx *ctx, long long n_cells, const float *c)
 /* centroids (n_cells*3) and per-species affiliation in upload order */
 int orbc_voronoi_get(orbc_ctx *ctx, float *centroids, int *aff_lipid, int *aff_protein);
 int orbc_voronoi_count(const orbc_ctx *ctx, long long *n_cells);
+/* delete_lipid (src/cleanup.h:29-91): in every Voronoi cell, removes the
+ * lipids whose squared distance to the centroid is not below tolerance^2
+ * times the cell's median one (element size/2 of the sorted distances), using
+ * the current cells.  orbc_run does the same every freq_cleanup steps on
+ * Voronoi-update steps (src/openrbc.cpp:200-202) while it keeps cells (volume
+ * constraint on, or freq_cleanup > 0 with cells initialised).  The lipid count
+ * shrinks (orbc_observables.n_lipid); orbc_get_state returns the remaining
+ * lipids in upload order.  *n_deleted = lipids removed by this call. */
+int orbc_delete_stray_lipids(orbc_ctx *ctx, double tolerance, long long *n_deleted);
 
 /* Device kernels behind reorder_morton / KDTree, exposed for parity checks:
  * morton_encode (src/reorder_morton.h:37-42) with bsize, on the GPU. */

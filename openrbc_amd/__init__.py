@@ -70,7 +70,8 @@ class Observables(C.Structure):
         ("virial", C.c_double), ("zeta", C.c_double), ("n", C.c_longlong), ("step", C.c_longlong),
         ("rebuilds", C.c_longlong), ("pairs", C.c_longlong), ("anchor_h", C.c_double), ("skin", C.c_double),
         ("anchor_lo", C.c_double * 3), ("anchor_spacing", C.c_double * 3), ("anchor_origin", C.c_double * 3),
-        ("anchor_bits", C.c_int * 3),
+        ("anchor_bits", C.c_int * 3), ("n_lipid", C.c_longlong), ("n_protein", C.c_longlong),
+        ("lipids_deleted", C.c_longlong),
     ]
 
 
@@ -120,6 +121,7 @@ SIGNATURES = [
     ("orbc_voronoi_set_centroids", C.c_int, [_vp, C.c_longlong, _f]),
     ("orbc_voronoi_get", C.c_int, [_vp, _f, _i, _i]),
     ("orbc_voronoi_count", C.c_int, [_vp, C.POINTER(C.c_longlong)]),
+    ("orbc_delete_stray_lipids", C.c_int, [_vp, C.c_double, C.POINTER(C.c_longlong)]),
     ("orbc_morton_keys", C.c_int, [_vp, C.c_longlong, _f, C.c_double, _u]),
     ("orbc_sort_pairs", C.c_int, [_vp, C.c_longlong, _u, _u, C.c_int]),
     ("orbc_nearest", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, _i]),
@@ -300,7 +302,26 @@ class Simulation:
         return T.value
 
     def run(self, nsteps: int):
-        self._check(self.L.orbc_run(self.h, int(nsteps)))
+        try:
+            self._check(self.L.orbc_run(self.h, int(nsteps)))
+        finally:
+            self._sync_counts()
+
+    def _sync_counts(self):
+        """Species counts after the device may have removed stray lipids."""
+        o = Observables()
+        if self.L.orbc_get_observables(self.h, C.byref(o)) == 0:
+            self.n[LIPID], self.n[PROTEIN] = o.n_lipid, o.n_protein
+
+    def delete_stray_lipids(self, tolerance: float = 2.5) -> int:
+        """delete_lipid (src/cleanup.h:29-91) with the current Voronoi cells;
+        returns the number of lipids removed."""
+        nd = C.c_longlong()
+        try:
+            self._check(self.L.orbc_delete_stray_lipids(self.h, float(tolerance), C.byref(nd)))
+        finally:
+            self._sync_counts()
+        return nd.value
 
     def minimize(self, nsteps: int, dr_opt: float = 5e-2, dn_opt: float = 5e-2):
         self._check(self.L.orbc_minimize(self.h, int(nsteps), dr_opt, dn_opt))

@@ -7,6 +7,7 @@
 // path, the Verlet list, and per-particle bond lists.  The host keeps each
 // species' upload order so results come back in the caller's order.
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -214,11 +215,12 @@ void upload_groups(orbc_ctx *c) {
         c->grp_alt = c->mem.alloc<uint16_t>(n);
         c->sig_group = c->mem.alloc<double>(c->group_kbt.size() * NT + NT);
     }
-    std::vector<uint16_t> img(n, 0);
-    long long k = 0;
+    // upload-order image: lipid slots from 0, protein slots from pslot0
+    std::vector<uint16_t> img(c->pslot0 + c->np, 0);
     for (int s = 0; s < 2; ++s) {
         const Species &S = c->sp[s];
-        for (long long u = 0; u < S.n; ++u, ++k) img[k] = S.group.empty() ? 0 : S.group[u];
+        const long long base = s == 0 ? 0 : c->pslot0;
+        for (long long u = 0; u < S.n; ++u) img[base + u] = S.group.empty() ? 0 : S.group[u];
     }
     const size_t ng = c->group_kbt.size();
     std::vector<double> sig(ng * NT);
@@ -231,7 +233,7 @@ void upload_groups(orbc_ctx *c) {
     }
     // the staging buffer holds 3n doubles: ample for n u16 in upload order
     uint16_t *tmp = reinterpret_cast<uint16_t *>(c->stage);
-    ORBC_HIP_CHECK(hipMemcpyAsync(tmp, img.data(), 2 * n, hipMemcpyHostToDevice, c->stream));
+    ORBC_HIP_CHECK(hipMemcpyAsync(tmp, img.data(), 2 * img.size(), hipMemcpyHostToDevice, c->stream));
     launch_gather_u16(n, c->slot, tmp, c->grp, c->stream);
     ORBC_HIP_CHECK(hipMemcpyAsync(c->sig_group, sig.data(), 8 * sig.size(), hipMemcpyHostToDevice, c->stream));
     sync(c);
@@ -246,6 +248,7 @@ void commit(orbc_ctx *c) {
     c->nl = c->sp[0].n;
     c->np = c->sp[1].n;
     c->n = c->nl + c->np;
+    c->pslot0 = c->nl;
     c->types_present = 0;
     if (c->n <= 0) throw std::runtime_error("no particles uploaded");
     if (c->n >= (1LL << 30)) throw std::runtime_error("more than 2^30 particles");
@@ -310,7 +313,9 @@ void set_cutoffs(orbc_ctx *c, double cs) {
         double rmax = 0;
         for (int b = 0; b < NT; ++b) {
             const double rc = ff_max_cut(a, b);
-            cut2[a * NT + b] = rc > 0 ? (float)((rc + skin) * (rc + skin)) : -1.0f;
+            // + 1e-4: the build tests FP32 positions relative to a cell anchor
+            // (rounding < 1e-6), so the list stays a superset of the pairs within rc + skin
+            cut2[a * NT + b] = rc > 0 ? (float)((rc + skin + 1e-4) * (rc + skin + 1e-4)) : -1.0f;
             // search reach only towards types present in the system
             if (rc > 0 && (c->types_present >> b & 1u)) rmax = std::max(rmax, rc + skin);
         }
@@ -322,6 +327,42 @@ void set_cutoffs(orbc_ctx *c, double cs) {
         if ((c->types_present >> a & 1u) && rings[a] >= 2) c->far_rings = true;
     }
     set_list_cutoffs(cut2, rings, c->stream);
+}
+
+// gathers every per-particle array of the store through perm (n_out
+// entries: perm[i] is the old index of the new particle i)
+static void permute_store(orbc_ctx *c, const uint32_t *perm, long long n_out) {
+    hipStream_t s = c->stream;
+    PermuteArgs p{};
+    p.n = n_out; p.perm = perm;
+    const int nx = 1 - c->cur;
+    p.src3[0] = c->x[c->cur]; p.dst3[0] = c->x[nx];
+    p.src3[1] = c->nrm[c->cur]; p.dst3[1] = c->nrm[nx];
+    p.src3[2] = c->v; p.dst3[2] = c->v_alt;
+    p.src3[3] = c->o; p.dst3[3] = c->o_alt;
+    if (c->mixed()) {
+        p.n3 = 4;
+        p.srcf3[0] = c->f32; p.dstf3[0] = c->f32_alt;
+        p.srcf3[1] = c->t32; p.dstf3[1] = c->t32_alt;
+        p.nf3 = 2;
+    } else {
+        p.src3[4] = c->f64; p.dst3[4] = c->f64_alt;
+        p.src3[5] = c->t64; p.dst3[5] = c->t64_alt;
+        p.n3 = 6; p.nf3 = 0;
+    }
+    p.type_src = c->type; p.type_dst = c->type_alt;
+    p.id_src = c->id; p.id_dst = c->id_alt;
+    p.slot_src = c->slot; p.slot_dst = c->slot_alt;
+    p.aff_src = c->voro.aff; p.aff_dst = c->voro.aff_alt;
+    p.grp_src = c->grp; p.grp_dst = c->grp_alt;
+    launch_permute(p, s);
+    c->cur = nx;
+    std::swap(c->v, c->v_alt); std::swap(c->o, c->o_alt);
+    std::swap(c->f32, c->f32_alt); std::swap(c->t32, c->t32_alt);
+    std::swap(c->f64, c->f64_alt); std::swap(c->t64, c->t64_alt);
+    std::swap(c->type, c->type_alt); std::swap(c->id, c->id_alt); std::swap(c->slot, c->slot_alt);
+    if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
+    if (c->grp) std::swap(c->grp, c->grp_alt);
 }
 
 void rebuild(orbc_ctx *c) {
@@ -427,36 +468,7 @@ void rebuild(orbc_ctx *c) {
                      &alt);
     const uint32_t *perm = alt ? c->vals_alt : c->vals;
     // 5. permute the store
-    PermuteArgs p{};
-    p.n = n; p.perm = perm;
-    const int nx = 1 - c->cur;
-    p.src3[0] = c->x[c->cur]; p.dst3[0] = c->x[nx];
-    p.src3[1] = c->nrm[c->cur]; p.dst3[1] = c->nrm[nx];
-    p.src3[2] = c->v; p.dst3[2] = c->v_alt;
-    p.src3[3] = c->o; p.dst3[3] = c->o_alt;
-    if (c->mixed()) {
-        p.n3 = 4;
-        p.srcf3[0] = c->f32; p.dstf3[0] = c->f32_alt;
-        p.srcf3[1] = c->t32; p.dstf3[1] = c->t32_alt;
-        p.nf3 = 2;
-    } else {
-        p.src3[4] = c->f64; p.dst3[4] = c->f64_alt;
-        p.src3[5] = c->t64; p.dst3[5] = c->t64_alt;
-        p.n3 = 6; p.nf3 = 0;
-    }
-    p.type_src = c->type; p.type_dst = c->type_alt;
-    p.id_src = c->id; p.id_dst = c->id_alt;
-    p.slot_src = c->slot; p.slot_dst = c->slot_alt;
-    p.aff_src = c->voro.aff; p.aff_dst = c->voro.aff_alt;
-    p.grp_src = c->grp; p.grp_dst = c->grp_alt;
-    launch_permute(p, s);
-    c->cur = nx;
-    std::swap(c->v, c->v_alt); std::swap(c->o, c->o_alt);
-    std::swap(c->f32, c->f32_alt); std::swap(c->t32, c->t32_alt);
-    std::swap(c->f64, c->f64_alt); std::swap(c->t64, c->t64_alt);
-    std::swap(c->type, c->type_alt); std::swap(c->id, c->id_alt); std::swap(c->slot, c->slot_alt);
-    if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
-    if (c->grp) std::swap(c->grp, c->grp_alt);
+    permute_store(c, perm, n);
     launch_inverse(c->slot, c->inv, n, s);
     // 6. mirrors, anchors, build-time offsets
     launch_refresh(n, c->x[c->cur], c->nrm[c->cur], c->type, ag, c->pq[c->cur], c->rb, c->anc, s);
@@ -499,6 +511,45 @@ void rebuild(orbc_ctx *c) {
     c->moved = false;
     c->since_build = 0;
     c->rebuilds++;
+}
+
+// Removes the particles with del[i] != 0 (lipids only) from the store: the
+// kept ones keep their order, lipid upload slots close up (proteins keep
+// theirs, so bond data indexed by protein slot stay valid), the host copies
+// of the lipid species drop the same entries, and the list is rebuilt.
+void compact_store(orbc_ctx *c, const uint32_t *del, long long ndel) {
+    if (ndel <= 0) return;
+    const long long n = c->n, n_new = n - ndel;
+    hipStream_t s = c->stream;
+    // scratch: keys (slot flags), vals (their scan), keys_alt (keep), vals_alt (positions), inv (perm)
+    launch_compact(n, del, c->slot, (uint32_t)c->pslot0, c->keys, c->vals, c->keys_alt, c->vals_alt, c->inv,
+                   c->sort_temp, s);
+    std::vector<uint32_t> gone(c->nl);
+    if (c->nl > 0)
+        ORBC_HIP_CHECK(hipMemcpyAsync(gone.data(), c->keys, 4 * c->nl, hipMemcpyDeviceToHost, s));
+    permute_store(c, c->inv, n_new);
+    sync(c);
+    Species &L = c->sp[ORBC_LIPID];
+    long long w = 0;
+    for (long long u = 0; u < L.n; ++u) {
+        if (gone[u]) continue;
+        for (int d = 0; d < 3; ++d) {
+            L.x[3 * w + d] = L.x[3 * u + d]; L.v[3 * w + d] = L.v[3 * u + d];
+            L.nrm[3 * w + d] = L.nrm[3 * u + d]; L.o[3 * w + d] = L.o[3 * u + d];
+        }
+        L.type[w] = L.type[u]; L.id[w] = L.id[u];
+        if (!L.group.empty()) L.group[w] = L.group[u];
+        ++w;
+    }
+    if (w != L.n - ndel) throw std::runtime_error("stray cleanup: inconsistent lipid count");
+    L.n = w;
+    L.x.resize(3 * w); L.v.resize(3 * w); L.nrm.resize(3 * w); L.o.resize(3 * w);
+    L.type.resize(w); L.id.resize(w);
+    if (!L.group.empty()) L.group.resize(w);
+    c->n = n_new;
+    c->nl -= ndel;
+    c->deleted += ndel;
+    rebuild(c);
 }
 
 void ensure_live(orbc_ctx *c) {
@@ -634,7 +685,7 @@ int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const
         S.id.assign(id, id + n);
         if (same && n > 0) {
             // update in place: upload in slot order, gather into device order
-            const long long off = species == ORBC_LIPID ? 0 : c->nl;
+            const long long off = species == ORBC_LIPID ? 0 : c->pslot0;
             double *fields[4] = {c->x[c->cur], c->v, c->nrm[c->cur], c->o};
             const std::vector<double> *src[4] = {&S.x, &S.v, &S.nrm, &S.o};
             for (int f = 0; f < 4; ++f) {
@@ -700,7 +751,7 @@ int orbc_get_state(orbc_ctx *c, int species, double *x, double *v, double *nrm, 
             if (o) std::memcpy(o, S.o.data(), 24 * S.n);
             return (int)ORBC_OK;
         }
-        const long long off = species == ORBC_LIPID ? 0 : c->nl;
+        const long long off = species == ORBC_LIPID ? 0 : c->pslot0;
         const long long n = species == ORBC_LIPID ? c->nl : c->np;
         const double *fields[4] = {c->x[c->cur], c->v, c->nrm[c->cur], c->o};
         double *outs[4] = {x, v, nrm, o};
@@ -718,7 +769,7 @@ int orbc_get_forces(orbc_ctx *c, int species, double *f, double *t, double *e, d
     return guarded(c, [&] {
         if (species != ORBC_LIPID && species != ORBC_PROTEIN) return fail(c, ORBC_ERR_INVALID, "bad species");
         if (!c->live) return fail(c, ORBC_ERR_STATE, "no device state");
-        const long long off = species == ORBC_LIPID ? 0 : c->nl;
+        const long long off = species == ORBC_LIPID ? 0 : c->pslot0;
         const long long n = species == ORBC_LIPID ? c->nl : c->np;
         if (n == 0) return (int)ORBC_OK;
         for (int k = 0; k < 2; ++k) {
@@ -744,7 +795,7 @@ int orbc_set_forces(orbc_ctx *c, int species, const double *f, const double *t) 
     return guarded(c, [&] {
         if (species != ORBC_LIPID && species != ORBC_PROTEIN) return fail(c, ORBC_ERR_INVALID, "bad species");
         ensure_live(c);
-        const long long off = species == ORBC_LIPID ? 0 : c->nl;
+        const long long off = species == ORBC_LIPID ? 0 : c->pslot0;
         const long long n = species == ORBC_LIPID ? c->nl : c->np;
         if (n == 0) return (int)ORBC_OK;
         for (int k = 0; k < 2; ++k) {
@@ -881,22 +932,45 @@ static void ev_pair(orbc_ctx *c, size_t k) {
 
 static bool volume_on(const orbc_ctx *c) { return c->cfg.volume_target != 0.0 && c->cfg.volume_strength != 0.0; }
 
-// Voronoi update every freq_voronoi steps (src/openrbc.cpp:200-205) and the
-// volume evaluation of constrain_volume (src/constrain_volume.h:26-83) before
-// the force evaluation of step `step`.  Inside a batch every kernel of it is
-// gated on the step's own skip condition, and the fused step kernel applies
-// the push; otherwise (to_f) the push is added to the force accumulators.
-static void volume_prestep(orbc_ctx *c, long long step, bool gated, bool to_f, bool cells = true) {
+// Voronoi update every freq_voronoi steps (src/openrbc.cpp:200-205) and, with
+// the volume constraint on, the volume evaluation of constrain_volume
+// (src/constrain_volume.h:26-83) before the force evaluation of step `step`.
+// Inside a batch every kernel of it is gated on the step's own skip
+// condition, and the fused step kernel applies the push; otherwise (to_f) the
+// push is added to the force accumulators.
+static void cell_prestep(orbc_ctx *c, long long step, bool gated, bool to_f, bool update_cells = true) {
     if (gated) {
         c->voro.gate.stop = c->iflags;
         c->voro.gate.slot = c->flags + 4 + (int)((step + 2) % 3);
     }
-    if (cells && step % c->cfg.freq_voronoi == 0) voronoi_update_dev(c);
-    if (to_f)
-        volume_push(c, c->vol_target, c->cfg.volume_strength, nullptr, true, false);
-    else
-        volume_eval(c, c->vol_target, c->cfg.volume_strength);
+    if (update_cells && step % c->cfg.freq_voronoi == 0) voronoi_update_dev(c);
+    if (volume_on(c)) {
+        if (to_f)
+            volume_push(c, c->vol_target, c->cfg.volume_strength, nullptr, true, false);
+        else
+            volume_eval(c, c->vol_target, c->cfg.volume_strength);
+    }
     c->voro.gate = Gate{};
+}
+
+// orbc_run keeps Voronoi cells current when they exist and something uses
+// them (the volume constraint, the stray-lipid cleanup)
+static bool cells_in_run(const orbc_ctx *c) {
+    return c->voro.ncell > 0 && (volume_on(c) || c->cfg.freq_cleanup > 0);
+}
+
+// delete_lipid runs on Voronoi-update steps that are multiples of
+// freq_cleanup, before the update (src/openrbc.cpp:200-202)
+static long long cleanup_period(const orbc_ctx *c) {
+    if (!cells_in_run(c) || c->cfg.freq_cleanup <= 0) return 0;
+    return std::lcm((long long)c->cfg.freq_voronoi, (long long)c->cfg.freq_cleanup);
+}
+
+static void cleanup_if_due(orbc_ctx *c) {
+    const long long L = cleanup_period(c);
+    if (L <= 0 || c->step % L != 0 || c->last_cleanup == c->step) return;
+    delete_stray_lipids(c, c->cfg.stray_tolerance);
+    c->last_cleanup = c->step;
 }
 
 static void volume_begin(orbc_ctx *c) {
@@ -913,8 +987,13 @@ static void volume_begin(orbc_ctx *c) {
 static void run_langevin(orbc_ctx *c, long long nsteps) {
     long long remaining = nsteps;
     const bool vol = volume_on(c);
+    const bool cells = cells_in_run(c);
+    const long long cperiod = cleanup_period(c);
     while (remaining > 0) {
+        cleanup_if_due(c);
         long long m = std::min<long long>(c->batch, remaining);
+        // a batch ends before the next cleanup step (it needs the host)
+        if (cperiod > 0) m = std::min<long long>(m, (c->step / cperiod + 1) * cperiod - c->step);
         if (c->cfg.rebuild_period > 0) {
             m = std::min<long long>(m, std::max<long long>(1, c->cfg.rebuild_period - c->since_build));
         } else if (c->list_life > 0) {
@@ -929,7 +1008,7 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
         const int cur0 = c->cur;
         if (c->timing) ev_pair(c, (size_t)m);
         for (long long s = 0; s < m; ++s) {
-            if (vol) volume_prestep(c, c->step + s, true, false);
+            if (cells) cell_prestep(c, c->step + s, true, false);
             StepArgs a = step_args(c);
             a.step = (unsigned long long)(c->step + s);
             if (vol) {
@@ -1004,13 +1083,17 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 return fail(c, ORBC_ERR_INVALID, "ensemble members with their own kBT need the Langevin thermostat");
             // Nose-Hoover, src/openrbc.cpp:176-243 with FUSED_INTEGRATOR
             IntegArgs a;
-            const double Q = nh_Q(c), nkT = 1.5 * (double)c->n * c->cfg.kBT;
-            const long long nb = reduce_blocks(c->n);
-            auto forces = [&](bool cells) {
+            // N can shrink inside the loop (stray-lipid cleanup): 1.5 N kT and the
+            // reduction width follow it
+            const double Q = nh_Q(c);
+            double nkT = 1.5 * (double)c->n * c->cfg.kBT;
+            long long nb = reduce_blocks(c->n);
+            const bool cells = cells_in_run(c);
+            auto forces = [&](bool update) {
                 clear_ft(c, false);
                 ForceArgs fa = force_args(c);
                 launch_forces(fa, c->mixed(), false, true, c->nbonds > 0, c->stream);
-                if (volume_on(c)) volume_prestep(c, c->step, false, true, cells);
+                if (cells) cell_prestep(c, c->step, false, true, update);
             };
             if (!c->nh_started) {
                 forces(false);
@@ -1027,6 +1110,9 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 c->since_build++;
                 bool rb = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
                 if (rb) { rebuild(c); } else { ensure_list(c); }
+                cleanup_if_due(c);
+                nkT = 1.5 * (double)c->n * c->cfg.kBT;
+                nb = reduce_blocks(c->n);
                 forces(true);
                 a = integ_args(c);
                 launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
@@ -1093,6 +1179,9 @@ int orbc_get_observables(orbc_ctx *c, orbc_observables *obs) {
         obs->potential = s[2];
         obs->virial = s[3];
         obs->n = c->n;
+        obs->n_lipid = c->live ? c->nl : c->sp[ORBC_LIPID].n;
+        obs->n_protein = c->live ? c->np : c->sp[ORBC_PROTEIN].n;
+        obs->lipids_deleted = c->deleted;
         obs->step = c->step;
         obs->rebuilds = c->rebuilds;
         // directed list entries

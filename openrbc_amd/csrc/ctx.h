@@ -65,6 +65,9 @@ struct orbc_ctx {
     // device store
     bool live = false;
     long long n = 0, nl = 0, np = 0, stride = 0;
+    long long pslot0 = 0;         // first protein slot (upload order: lipids, then proteins)
+    long long last_cleanup = -1;  // step of the last stray-lipid cleanup inside orbc_run
+    long long deleted = 0;        // lipids removed by the cleanup so far
     double *x[2] = {nullptr, nullptr}, *nrm[2] = {nullptr, nullptr};
     int cur = 0;
     double *v = nullptr, *o = nullptr, *v_alt = nullptr, *o_alt = nullptr;
@@ -134,6 +137,8 @@ struct orbc_ctx {
         float *cref = nullptr;             // centroids as binned in the centroid grid
         unsigned *cdisp = nullptr;         // max squared drift since (float bits)
         Gate gate;                         // skip gate of the batched run (VoroArgs::gate)
+        // stray-lipid cleanup scratch: member ranges per cell key, delete flags
+        uint32_t *st_start = nullptr, *st_end = nullptr, *del = nullptr;
     } voro;
     double list_life = 0.0;       // steps a neighbour list lasts on average (batch sizing)
     // ensemble members (orbc_set_groups): per-particle member and per-member kBT
@@ -162,6 +167,10 @@ void rebuild(orbc_ctx *c);
 void voronoi_free(orbc_ctx *c);
 void voronoi_rebuild_index(orbc_ctx *c);
 void voronoi_update_dev(orbc_ctx *c);
+// stray-lipid cleanup (src/cleanup.h:29-91) with the current cells; removes
+// the strays from the store and rebuilds.  Returns the number removed.
+long long delete_stray_lipids(orbc_ctx *c, double tolerance);
+void compact_store(orbc_ctx *c, const uint32_t *del, long long ndel);
 double volume_push(orbc_ctx *c, double target, double strength, double *f_ext, bool to_f, bool read_back);
 void volume_eval(orbc_ctx *c, double target, double strength);
 }  // namespace orbc

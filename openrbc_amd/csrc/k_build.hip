@@ -138,6 +138,28 @@ __global__ __launch_bounds__(BLOCK) void k_permute(PermuteArgs a) {
     if (a.grp_src) a.grp_dst[i] = a.grp_src[p];
 }
 
+// ---- removal of particles (stray-lipid cleanup) -------------------------------
+// flag[slot] = 1 for the deleted particles (flag zeroed on entry); keep[i] = !del[i]
+__global__ __launch_bounds__(BLOCK) void k_del_flags(long long n, const uint32_t *slot, const uint32_t *del,
+                                                     uint32_t *flag, uint32_t *keep) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (del[i]) flag[slot[i]] = 1u;
+    keep[i] = del[i] ? 0u : 1u;
+}
+
+// kept particle i goes to pos[i]; its slot, if below pslot0 (a lipid), closes
+// the gaps of the deleted slots below it (slot_pre: exclusive scan of flag)
+__global__ __launch_bounds__(BLOCK) void k_compact(long long n, const uint32_t *del, const uint32_t *pos,
+                                                   uint32_t *perm, uint32_t *slot, const uint32_t *slot_pre,
+                                                   uint32_t pslot0) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n || del[i]) return;
+    perm[pos[i]] = (uint32_t)i;
+    const uint32_t sl = slot[i];
+    if (sl < pslot0) slot[i] = sl - slot_pre[sl];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_inverse(const uint32_t *slot, uint32_t *inv, long long n) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) inv[slot[i]] = (uint32_t)i;
@@ -357,7 +379,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell_compact(const uint32_t *flag, co
 // sigma-3.4 LJ) get the outer shells appended by k_nlist_far afterwards.
 __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *occ, long long nocc) {
     __shared__ float4 s_x[BLOCK / 64][NL_CAP];
-    __shared__ uint32_t s_j[BLOCK / 64][NL_CAP];
     __shared__ uint32_t s_pref[BLOCK / 64][28];
     __shared__ uint32_t s_cst[BLOCK / 64][27];
     __shared__ float s_cut2[36];
@@ -397,6 +418,16 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
     wave_sync();
     const long long np = p1 - p0;
     const Anchor &an = a.anc;
+    // positions relative to the anchor of the cell's first particle, formed
+    // once per particle / staged candidate: the pair test is then 3 subtracts
+    const uint32_t wref = __float_as_uint(a.pq[p0].x.w);
+    const int rx = (int)(wref & an.mx), ry = (int)((wref >> an.shy) & an.my), rz = (int)(wref >> an.shz);
+    auto rel = [&](float4 q) {
+        const uint32_t wq = __float_as_uint(q.w);
+        return make_float3(anchor_add(an, 0, q.x, (int)(wq & an.mx) - rx),
+                           anchor_add(an, 1, q.y, (int)((wq >> an.shy) & an.my) - ry),
+                           anchor_add(an, 2, q.z, (int)(wq >> an.shz) - rz));
+    };
     for (long long pc = 0; pc < np; pc += 64) {
         const int npc = (int)min(64LL, np - pc);
         int P = 1;
@@ -406,16 +437,13 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
         const bool act = pslot < npc;
         const long long i = p0 + pc + (act ? pslot : 0);
         NlCtx c{a, i, 0, make_float4(0, 0, 0, 0), 0, 0, 0, 0.f, 0.f, 0.f, 0};
-        float crow[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) crow[t] = -1.f;
+        float3 pi = make_float3(0.f, 0.f, 0.f);
+        int crow = 0;                       // this particle's row of the cutoff table
         if (act) {
             c.ti = a.type[i];
             c.xi = a.pq[i].x;
-            const uint32_t wi = __float_as_uint(c.xi.w);
-            c.ax = (int)(wi & an.mx); c.ay = (int)((wi >> an.shy) & an.my); c.az = (int)(wi >> an.shz);
-#pragma unroll
-            for (int t = 0; t < NT; ++t) crow[t] = s_cut2[c.ti * NT + t];
+            pi = rel(c.xi);
+            crow = c.ti * NT;
             if (a.aff) {
                 const int ca = a.aff[i];
                 c.c0 = a.centroids[3 * ca]; c.c1 = a.centroids[3 * ca + 1]; c.c2 = a.centroids[3 * ca + 2];
@@ -436,25 +464,20 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
                     if (s_pref[w][mid] <= gidx) lo = mid; else hi = mid;
                 }
                 const uint32_t j = s_cst[w][lo] + (gidx - s_pref[w][lo]);
-                s_x[w][t] = a.pq[j].x;
-                s_j[w][t] = j | ((uint32_t)a.type[j] << 29);
+                const float3 pj = rel(a.pq[j].x);
+                s_x[w][t] = make_float4(pj.x, pj.y, pj.z, __uint_as_float(j | ((uint32_t)a.type[j] << 29)));
             }
             wave_sync();
             for (uint32_t t0 = 0; t0 < m; t0 += S) {
                 const uint32_t t = t0 + stripe;
                 const bool valid = act && t < m;
                 const uint32_t tt = valid ? t : 0u;
-                const uint32_t jt = s_j[w][tt];
                 const float4 xj = s_x[w][tt];
+                const uint32_t jt = __float_as_uint(xj.w);
                 const uint32_t j = jt & 0x1fffffffu;
                 const int tj = (int)(jt >> 29);
-                float cut2 = crow[0];
-#pragma unroll
-                for (int q2 = 1; q2 < NT; ++q2) cut2 = (tj == q2) ? crow[q2] : cut2;
-                const uint32_t wj = __float_as_uint(xj.w);
-                const float dx = (c.xi.x - xj.x) + anchor_dx(an, 0, c.ax - (int)(wj & an.mx));
-                const float dy = (c.xi.y - xj.y) + anchor_dx(an, 1, c.ay - (int)((wj >> an.shy) & an.my));
-                const float dz = (c.xi.z - xj.z) + anchor_dx(an, 2, c.az - (int)(wj >> an.shz));
+                const float cut2 = s_cut2[crow + tj];
+                const float dx = pi.x - xj.x, dy = pi.y - xj.y, dz = pi.z - xj.z;
                 // a pair without interaction has cut2 < 0 and never passes
                 bool hit = valid && j != (uint32_t)i && dx * dx + dy * dy + dz * dz < cut2;
                 if (a.aff && hit) {
@@ -594,6 +617,17 @@ void launch_keys(const double *x, long long n, Grid g, uint32_t *keys, uint32_t 
 void launch_permute(const PermuteArgs &a, hipStream_t s) {
     if (a.n <= 0) return;
     hipLaunchKernelGGL(k_permute, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a);
+}
+
+void launch_compact(long long n, const uint32_t *del, uint32_t *slot, uint32_t pslot0, uint32_t *flag,
+                    uint32_t *slot_pre, uint32_t *keep, uint32_t *pos, uint32_t *perm, void *scan_temp,
+                    hipStream_t s) {
+    if (n <= 0) return;
+    launch_fill_u32(flag, 0u, n, s);
+    hipLaunchKernelGGL(k_del_flags, dim3(nblk(n)), dim3(BLOCK), 0, s, n, slot, del, flag, keep);
+    exclusive_scan_u32(flag, slot_pre, n, scan_temp, s);
+    exclusive_scan_u32(keep, pos, n, scan_temp, s);
+    hipLaunchKernelGGL(k_compact, dim3(nblk(n)), dim3(BLOCK), 0, s, n, del, pos, perm, slot, slot_pre, pslot0);
 }
 
 void launch_inverse(const uint32_t *slot, uint32_t *inv, long long n, hipStream_t s) {

@@ -269,16 +269,22 @@ __device__ inline void add_to(Acc<double> &o, const Acc<R> &a) {
     o.e += (double)a.e; o.w += (double)a.w;
 }
 
-// Tuning knobs (compile-time): neighbours gathered per group (memory-level
-// parallelism per wave) and, if nonzero, the waves per SIMD the fused step
-// kernel's register budget is capped for.
+// Tuning knobs (compile-time): neighbours gathered per group in the mixed
+// and FP64 walks (memory-level parallelism per wave; measured on the full
+// RBC: mixed 3 with the pipelined gathers below, 0.574 ms against 0.594 for 2
+// and 0.613 for 4 unpipelined) and, if nonzero, the waves per SIMD the fused
+// step kernel's register budget is capped for.
 #ifndef ORBC_STEP_UNROLL
-#define ORBC_STEP_UNROLL 4
+#define ORBC_STEP_UNROLL 3
+#endif
+#ifndef ORBC_FP64_UNROLL
+#define ORBC_FP64_UNROLL 4
 #endif
 #ifndef ORBC_STEP_WAVES
 #define ORBC_STEP_WAVES 0
 #endif
 constexpr int UNROLL = ORBC_STEP_UNROLL;
+constexpr int UNROLL64 = ORBC_FP64_UNROLL;
 #if ORBC_STEP_WAVES > 0
 #define ORBC_STEP_ATTR __attribute__((amdgpu_waves_per_eu(ORBC_STEP_WAVES)))
 #else
@@ -293,6 +299,31 @@ constexpr int UNROLL = ORBC_STEP_UNROLL;
 // Mixed: pair terms from the FP32 mirrors, summed in FP64 (ORBC_PAIR_F64 >= 2)
 // or in FP32 over each group and in FP64 across groups.
 using PairAcc = std::conditional<(ORBC_PAIR_F64 >= 2), double, float>::type;
+// one gathered neighbour against owner i (mixed path)
+template <bool ENERGY, typename AR>
+__device__ __forceinline__ void pair_mixed(const Anchor &ag, const AnchorI &ai, float4 xi, float4 ni, int ti,
+                                           float4 xj, float4 nj, const FFS<float> &t, Acc<AR> &acc) {
+    const AnchorI aj = unpack(__float_as_uint(xj.w), ag);
+    const float dx = anchor_add(ag, 0, sub_rn(xi.x, xj.x), ai.x - aj.x);
+    const float dy = anchor_add(ag, 1, sub_rn(xi.y, xj.y), ai.y - aj.y);
+    const float dz = anchor_add(ag, 2, sub_rn(xi.z, xj.z), ai.z - aj.z);
+    const float r2 = r2_rn(dx, dy, dz);
+    double X = 0.0, Y = 0.0, Z = 0.0;
+    if constexpr (ORBC_PAIR_F64 >= 2) {
+        X = anchor_add_exact(ag, 0, xi.x, xj.x, ai.x - aj.x);
+        Y = anchor_add_exact(ag, 1, xi.y, xj.y, ai.y - aj.y);
+        Z = anchor_add_exact(ag, 2, xi.z, xj.z, ai.z - aj.z);
+    }
+    pair_ij<float, ENERGY>(t, ti, __float_as_int(nj.w), dx, dy, dz, X, Y, Z, r2, ni.x, ni.y, ni.z, nj.x, nj.y, nj.z,
+                           acc);
+}
+
+// ORBC_PIPE: the mirrors of the next group are gathered while this group is
+// evaluated (two groups of UNROLL records in registers), instead of
+// gathering each group right before its use.
+#ifndef ORBC_PIPE
+#define ORBC_PIPE 1
+#endif
 template <bool ENERGY>
 __device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long long stride, const PQ *pq,
                                             const uint32_t *nb, const Anchor &ag, const AnchorI &ai, float4 xi,
@@ -300,39 +331,60 @@ __device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long lo
     uint32_t jn[UNROLL];
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) jn[u] = (kb + u < ke) ? nb[(long long)(kb + u) * stride] : (uint32_t)i;
-    for (int k0 = kb; k0 < ke; k0 += UNROLL) {
-        uint32_t jj[UNROLL];
+    if constexpr (ORBC_PIPE) {
+        float4 xa[UNROLL], na[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) { xa[u] = pq[jn[u]].x; na[u] = pq[jn[u]].n; }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            jj[u] = jn[u];
-            const int kn = k0 + UNROLL + u;
+            const int kn = kb + UNROLL + u;
             jn[u] = (kn < ke) ? nb[(long long)kn * stride] : (uint32_t)i;
         }
-        float4 xj[UNROLL], nj[UNROLL];
+        for (int k0 = kb; k0 < ke; k0 += UNROLL) {
+            float4 xb[UNROLL], nbv[UNROLL];
+            const bool more = k0 + UNROLL < ke;
+            if (more) {
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) { xj[u] = pq[jj[u]].x; nj[u] = pq[jj[u]].n; }
-        Acc<float> acc32;
-        // FP64 terms go straight into the FP64 sums
-        Acc<PairAcc> &acc = *[&]() {
-            if constexpr (ORBC_PAIR_F64 >= 2) return &out; else return &acc32;
-        }();
+                for (int u = 0; u < UNROLL; ++u) { xb[u] = pq[jn[u]].x; nbv[u] = pq[jn[u]].n; }
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-            const AnchorI aj = unpack(__float_as_uint(xj[u].w), ag);
-            const float dx = anchor_add(ag, 0, sub_rn(xi.x, xj[u].x), ai.x - aj.x);
-            const float dy = anchor_add(ag, 1, sub_rn(xi.y, xj[u].y), ai.y - aj.y);
-            const float dz = anchor_add(ag, 2, sub_rn(xi.z, xj[u].z), ai.z - aj.z);
-            const float r2 = r2_rn(dx, dy, dz);
-            double X = 0.0, Y = 0.0, Z = 0.0;
-            if constexpr (ORBC_PAIR_F64 >= 2) {
-                X = anchor_add_exact(ag, 0, xi.x, xj[u].x, ai.x - aj.x);
-                Y = anchor_add_exact(ag, 1, xi.y, xj[u].y, ai.y - aj.y);
-                Z = anchor_add_exact(ag, 2, xi.z, xj[u].z, ai.z - aj.z);
+                for (int u = 0; u < UNROLL; ++u) {
+                    const int kn = k0 + 2 * UNROLL + u;
+                    jn[u] = (kn < ke) ? nb[(long long)kn * stride] : (uint32_t)i;
+                }
             }
-            pair_ij<float, ENERGY>(t, ti, __float_as_int(nj[u].w), dx, dy, dz, X, Y, Z, r2, ni.x, ni.y, ni.z,
-                                   nj[u].x, nj[u].y, nj[u].z, acc);
+            Acc<float> acc32;
+            Acc<PairAcc> &acc = *[&]() {
+                if constexpr (ORBC_PAIR_F64 >= 2) return &out; else return &acc32;
+            }();
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xa[u], na[u], t, acc);
+            if constexpr (ORBC_PAIR_F64 < 2) add_to(out, acc32);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) { xa[u] = xb[u]; na[u] = nbv[u]; }
+            }
         }
-        if constexpr (ORBC_PAIR_F64 < 2) add_to(out, acc32);
+    } else {
+        for (int k0 = kb; k0 < ke; k0 += UNROLL) {
+            uint32_t jj[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                jj[u] = jn[u];
+                const int kn = k0 + UNROLL + u;
+                jn[u] = (kn < ke) ? nb[(long long)kn * stride] : (uint32_t)i;
+            }
+            float4 xj[UNROLL], nj[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) { xj[u] = pq[jj[u]].x; nj[u] = pq[jj[u]].n; }
+            Acc<float> acc32;
+            // FP64 terms go straight into the FP64 sums
+            Acc<PairAcc> &acc = *[&]() {
+                if constexpr (ORBC_PAIR_F64 >= 2) return &out; else return &acc32;
+            }();
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xj[u], nj[u], t, acc);
+            if constexpr (ORBC_PAIR_F64 < 2) add_to(out, acc32);
+        }
     }
 }
 
@@ -370,26 +422,26 @@ __device__ __forceinline__ void range_fp64(long long i, int kb, int ke, long lon
                                            const Anchor &ag, bool periodic, double xi,
                                            double yi, double zi, double nix, double niy, double niz, int ti,
                                            const FFS<double> &t, Acc<double> &acc) {
-    long long jn[UNROLL];
+    long long jn[UNROLL64];
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) jn[u] = (kb + u < ke) ? (long long)nb[(long long)(kb + u) * stride] : i;
-    for (int k0 = kb; k0 < ke; k0 += UNROLL) {
-        long long jj[UNROLL];
+    for (int u = 0; u < UNROLL64; ++u) jn[u] = (kb + u < ke) ? (long long)nb[(long long)(kb + u) * stride] : i;
+    for (int k0 = kb; k0 < ke; k0 += UNROLL64) {
+        long long jj[UNROLL64];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
+        for (int u = 0; u < UNROLL64; ++u) {
             jj[u] = jn[u];
-            const int kn = k0 + UNROLL + u;
+            const int kn = k0 + UNROLL64 + u;
             jn[u] = (kn < ke) ? (long long)nb[(long long)kn * stride] : i;
         }
-        double xj[UNROLL][3], nj[UNROLL][3];
-        int tj[UNROLL];
+        double xj[UNROLL64][3], nj[UNROLL64][3];
+        int tj[UNROLL64];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
+        for (int u = 0; u < UNROLL64; ++u) {
             for (int d = 0; d < 3; ++d) { xj[u][d] = x[3 * jj[u] + d]; nj[u][d] = nrm[3 * jj[u] + d]; }
             tj[u] = type[jj[u]];
         }
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
+        for (int u = 0; u < UNROLL64; ++u) {
             double dx = sub_rn(xi, xj[u][0]);
             double dy = sub_rn(yi, xj[u][1]);
             double dz = sub_rn(zi, xj[u][2]);

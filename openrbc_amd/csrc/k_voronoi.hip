@@ -499,6 +499,62 @@ __global__ void k_volume_force(const double *vol_partial, long long nb, double t
     }
 }
 
+// ---- stray-lipid cleanup, src/cleanup.h:29-91 --------------------------------
+// key: 1 + the lipid's cell (0 for proteins and lipids without one), so a
+// stable sort by key groups the members of every cell; d2 = |x - c|^2 in FP64
+// against the FP32 centroid.
+__global__ __launch_bounds__(BLOCK) void k_stray_keys(VoroArgs a, uint32_t *keys, uint32_t *vals, double *d2) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    int k = a.type[i] == 0 ? a.aff[i] : -1;
+    double d = 0.0;
+    if (k >= 0 && k < a.ncell) {
+        const double dx = a.x[3 * i] - (double)a.centroids[3 * k];
+        const double dy = a.x[3 * i + 1] - (double)a.centroids[3 * k + 1];
+        const double dz = a.x[3 * i + 2] - (double)a.centroids[3 * k + 2];
+        d = __fma_rn(dz, dz, __fma_rn(dy, dy, dx * dx));
+    } else {
+        k = -1;
+    }
+    keys[i] = (uint32_t)(k + 1);
+    vals[i] = (uint32_t)i;
+    d2[i] = d;
+}
+
+// One thread per cell: the threshold is the member distance of rank size/2
+// in ascending order (dr2[size / 2] after the reference's sort) times tol^2,
+// and a member is deleted unless d2 < threshold.  del[] is written for every
+// member; the count of deletions is added to *count.
+__global__ __launch_bounds__(BLOCK) void k_stray_mark(long long ncell, const uint32_t *start, const uint32_t *end,
+                                                      const uint32_t *svals, const double *d2, double tol2,
+                                                      uint32_t *del, unsigned *count) {
+    const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= ncell) return;
+    const uint32_t b = start[k + 1], e = end[k + 1];
+    if (e <= b) return;
+    const uint32_t half = (e - b) / 2;
+    double med = 0.0;
+    for (uint32_t p = b; p < e; ++p) {
+        const double v = d2[svals[p]];
+        uint32_t lt = 0, le = 0;
+        for (uint32_t q = b; q < e; ++q) {
+            const double w = d2[svals[q]];
+            lt += w < v;
+            le += w <= v;
+        }
+        if (lt <= half && half < le) { med = v; break; }
+    }
+    const double thr = med * tol2;
+    unsigned nd = 0;
+    for (uint32_t p = b; p < e; ++p) {
+        const uint32_t i = svals[p];
+        const bool kill = !(d2[i] < thr);
+        del[i] = kill ? 1 : 0;
+        nd += kill;
+    }
+    if (nd) atomicAdd(count, nd);
+}
+
 }  // namespace
 
 void launch_morton_ref(const float *pts, long long n, double bsize, uint32_t *keys, hipStream_t s) {
@@ -563,6 +619,18 @@ void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals,
 void launch_volume_push(const VoroArgs &a, const double *normals, const double *fpush, double *f_ext, double *f64,
                         float *f32, hipStream_t s) {
     if (a.n > 0) hipLaunchKernelGGL(k_volume_push, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a, normals, fpush, f_ext, f64, f32);
+}
+
+
+void launch_stray_keys(const VoroArgs &a, uint32_t *keys, uint32_t *vals, double *d2, hipStream_t s) {
+    if (a.n > 0) hipLaunchKernelGGL(k_stray_keys, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a, keys, vals, d2);
+}
+
+void launch_stray_mark(long long ncell, const uint32_t *start, const uint32_t *end, const uint32_t *svals,
+                       const double *d2, double tol2, uint32_t *del, unsigned *count, hipStream_t s) {
+    if (ncell > 0)
+        hipLaunchKernelGGL(k_stray_mark, dim3(nblk(ncell)), dim3(BLOCK), 0, s, ncell, start, end, svals, d2, tol2,
+                           del, count);
 }
 
 }  // namespace orbc

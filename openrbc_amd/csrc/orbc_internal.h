@@ -214,6 +214,12 @@ struct PermuteArgs {
     const uint16_t *grp_src; uint16_t *grp_dst;         // ensemble member (or null)
 };
 void launch_permute(const PermuteArgs &a, hipStream_t s);
+// removal of the particles with del[i] != 0: perm[0..n - ndel) lists the kept
+// particles in store order; lipid slots (< pslot0) are renumbered without
+// gaps (flag, slot_pre, keep, pos: n-entry scratch)
+void launch_compact(long long n, const uint32_t *del, uint32_t *slot, uint32_t pslot0, uint32_t *flag,
+                    uint32_t *slot_pre, uint32_t *keep, uint32_t *pos, uint32_t *perm, void *scan_temp,
+                    hipStream_t s);
 void launch_inverse(const uint32_t *slot, uint32_t *inv, long long n, hipStream_t s);
 void launch_refresh(long long n, const double *x, const double *nrm, const uint8_t *type, Anchor a,
                     PQ *pq, float *rb, uint32_t *anc, hipStream_t s);
@@ -318,6 +324,10 @@ void launch_centroid_keys(const float *c, long long ncell, Grid g, uint32_t *key
                           Gate gt = {});
 void launch_cgrid_ranges(const uint32_t *sorted_keys, long long ncell, uint32_t *start, uint32_t *end,
                          hipStream_t s, Gate gt = {});
+// stray-lipid cleanup (src/cleanup.h:29-91)
+void launch_stray_keys(const VoroArgs &a, uint32_t *keys, uint32_t *vals, double *d2, hipStream_t s);
+void launch_stray_mark(long long ncell, const uint32_t *start, const uint32_t *end, const uint32_t *svals,
+                       const double *d2, double tol2, uint32_t *del, unsigned *count, hipStream_t s);
 void launch_nearest(const VoroArgs &a, const float *q, long long nq, const double *qd, int32_t *out, bool warm,
                     hipStream_t s);
 void launch_within(const VoroArgs &a, const float *q, long long nq, float rmax, int cap, int *counts,

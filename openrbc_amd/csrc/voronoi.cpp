@@ -141,13 +141,44 @@ namespace orbc {
 void voronoi_free(orbc_ctx *c) {
     auto &V = c->voro;
     void *ps[] = {V.aff, V.aff_alt, V.c, V.sums, V.normals, V.partial, V.ck, V.cv, V.ck2, V.cv2, V.cg_start, V.cg_end,
-                  V.temp, V.cref, V.cdisp};
+                  V.temp, V.cref, V.cdisp, V.st_start, V.st_end, V.del};
     for (void *p : ps) c->mem.release(p);
     V = orbc_ctx::Voro{};
 }
 
 void voronoi_rebuild_index(orbc_ctx *c) {
     if (c->voro.ncell > 0) index_centroids(c);
+}
+
+// delete_lipid, src/cleanup.h:29-91: per cell, the lipids whose squared
+// distance to the centroid is not below tol^2 times the cell's median one
+// (dr2[size / 2] of the sorted distances) are removed.  Uses the current
+// centroids and affiliation, as the reference uses its current cell lists.
+long long delete_stray_lipids(orbc_ctx *c, double tolerance) {
+    auto &V = c->voro;
+    if (V.ncell <= 0 || !V.aff_valid || c->nl <= 0) return 0;
+    const long long n = c->n;
+    hipStream_t s = c->stream;
+    if (!V.st_start) {
+        V.st_start = c->mem.alloc<uint32_t>(V.ncell + 1);
+        V.st_end = c->mem.alloc<uint32_t>(V.ncell + 1);
+        V.del = c->mem.alloc<uint32_t>(c->stride);
+    }
+    VoroArgs a = voro_args(c);
+    double *d2 = c->stage;                        // 3n doubles of staging: n distances
+    launch_stray_keys(a, c->keys, c->vals, d2, s);
+    bool alt = false;
+    radix_sort_pairs(c->keys, c->vals, c->keys_alt, c->vals_alt, n, ceil_log2(V.ncell + 1), c->sort_temp, s, &alt);
+    const uint32_t *sk = alt ? c->keys_alt : c->keys, *sv = alt ? c->vals_alt : c->vals;
+    launch_fill_u32(V.st_start, 0u, V.ncell + 1, s);
+    launch_fill_u32(V.st_end, 0u, V.ncell + 1, s);
+    launch_cgrid_ranges(sk, n, V.st_start, V.st_end, s);
+    launch_fill_u32(V.del, 0u, n, s);
+    zero_flag(c, 3);
+    launch_stray_mark(V.ncell, V.st_start, V.st_end, sv, d2, tolerance * tolerance, V.del, c->flags + 3, s);
+    const long long ndel = read_flag(c, 3);
+    if (ndel > 0) compact_store(c, V.del, ndel);
+    return ndel;
 }
 
 // VoronoiDiagram::update + VCellList::update of both containers
@@ -256,11 +287,22 @@ int orbc_voronoi_get(orbc_ctx *c, float *cen, int *aff_lipid, int *aff_protein) 
             (void)d;
             for (long long i = 0; i < c->n; ++i) {
                 const long long s = hs[i];
-                if (s < c->nl) { if (aff_lipid) aff_lipid[s] = hv[i]; }
-                else if (aff_protein) aff_protein[s - c->nl] = hv[i];
+                if (s < c->pslot0) { if (aff_lipid) aff_lipid[s] = hv[i]; }
+                else if (aff_protein) aff_protein[s - c->pslot0] = hv[i];
             }
         }
         sync(c);
+        return (int)ORBC_OK;
+    });
+}
+
+int orbc_delete_stray_lipids(orbc_ctx *c, double tolerance, long long *n_deleted) {
+    return guarded(c, [&] {
+        ensure_live(c);
+        if (c->voro.ncell <= 0) return fail(c, ORBC_ERR_STATE, "orbc_voronoi_init first");
+        if (!(tolerance > 0)) return fail(c, ORBC_ERR_INVALID, "tolerance must be positive");
+        const long long nd = delete_stray_lipids(c, tolerance);
+        if (n_deleted) *n_deleted = nd;
         return (int)ORBC_OK;
     });
 }

@@ -25,6 +25,7 @@ _f = C.POINTER(C.c_float)
 _i32 = C.POINTER(C.c_int32)
 _u32 = C.POINTER(C.c_uint32)
 _i64 = C.POINTER(C.c_int64)
+_u8 = C.POINTER(C.c_uint8)
 
 
 def build(ref: bool = True) -> None:
@@ -98,6 +99,8 @@ class Oracle:
         L.orc_stencil.argtypes = [C.c_int64, _f, C.c_int64, C.c_float, C.c_int64, _i64]
         L.orc_pairs_voronoi_fp64.restype = C.c_int64
         L.orc_pairs_voronoi_fp64.argtypes = [C.c_int64, _d, _i32, _i32, C.c_int64, _f, C.c_int64, _i64]
+        L.orc_stray_delete.restype = C.c_int64
+        L.orc_stray_delete.argtypes = [C.c_int64, _d, _i32, _i32, C.c_int64, _f, C.c_double, _u8]
         L.orc_constrain_volume.restype = C.c_double
         L.orc_constrain_volume.argtypes = [C.c_int64, _d, _i32, _i32, C.c_int64, _f, C.c_double, C.c_double, _d]
         L.orc_set_box.restype = None
@@ -266,6 +269,14 @@ class Oracle:
         k = self.L.orc_stencil(len(c), _p(c, _f), int(cell), float(rmax), len(c), _p(out, _i64))
         return out[:k]
 
+    def stray_delete(self, x, type_, aff, centroids, tol):
+        """Boolean mask of the lipids delete_lipid removes (src/cleanup.h:29-60)."""
+        x = _c(x, np.float64); t = _c(type_, np.int32); a = _c(aff, np.int32); c = _c(centroids, np.float32)
+        d = np.zeros(len(t), np.uint8)
+        self.L.orc_stray_delete(len(t), _p(x, _d), _p(t, _i32), _p(a, _i32), len(c), _p(c, _f), float(tol),
+                                _p(d, _u8))
+        return d.astype(bool)
+
     def constrain_volume(self, nrm, type_, aff, centroids, target, strength, f):
         nrm = _c(nrm, np.float64); t = _c(type_, np.int32); a = _c(aff, np.int32)
         c = _c(centroids, np.float32)
@@ -298,6 +309,7 @@ class Reference:
         L.ref_kdtree_nearest.argtypes = [C.c_int, _f, C.c_int, _f, _i32]
         L.ref_kdtree_within.argtypes = [C.c_int, _f, C.c_int, _f, C.c_float, C.c_int, _i32, _i32]
         L.ref_partition.argtypes = [C.c_int, _f, C.c_int, _f, _i32]
+        L.ref_stray.argtypes = [C.c_int, _f, C.c_int, _f, C.c_double, _i32, _u8]
         L.ref_run.restype = C.c_double
         L.ref_run.argtypes = [C.c_int, _f, _f, C.c_int, _f, _f, _i32, _i32, C.c_int, _i32, _i32, _i32,
                               C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -377,3 +389,10 @@ class Reference:
         aff = np.zeros(len(x), np.int32)
         self.L.ref_partition(len(x), _p(x, _f), len(c), _p(c, _f), _p(aff, _i32))
         return aff
+
+    def stray(self, x, centroids, tol):
+        """delete_lipid on lipids x with fixed centroids: (cell of each lipid, survives)."""
+        x = _c(x, np.float32); c = _c(centroids, np.float32)
+        aff = np.zeros(len(x), np.int32); keep = np.zeros(len(x), np.uint8)
+        self.L.ref_stray(len(x), _p(x, _f), len(c), _p(c, _f), float(tol), _p(aff, _i32), _p(keep, _u8))
+        return aff, keep.astype(bool)

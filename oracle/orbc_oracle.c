@@ -693,3 +693,47 @@ double orc_constrain_volume(int64_t n, const double *nrm, const int32_t *type,
     free(nm); free(cnt);
     return vol;
 }
+
+static int cmp_double(const void *a, const void *b) {
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+int64_t orc_stray_delete(int64_t n, const double *x, const int32_t *type, const int32_t *aff, int64_t ncell,
+                         const float *c, double tol, uint8_t *del) {
+    double *d2 = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    int64_t *cnt = calloc((size_t)ncell + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < n; ++i) {
+        del[i] = 0;
+        const int32_t k = aff[i];
+        if (type[i] != 0 || k < 0 || k >= ncell) { d2[i] = 0.0; continue; }
+        const double dx = x[3 * i] - (double)c[3 * k], dy = x[3 * i + 1] - (double)c[3 * k + 1],
+                     dz = x[3 * i + 2] - (double)c[3 * k + 2];
+        d2[i] = dx * dx + dy * dy + dz * dz;
+        cnt[k + 1]++;
+    }
+    for (int64_t k = 0; k < ncell; ++k) cnt[k + 1] += cnt[k];          /* cell k: [cnt[k], cnt[k+1]) */
+    double *sorted = malloc((size_t)(cnt[ncell] > 0 ? cnt[ncell] : 1) * sizeof(double));
+    int64_t *fill = calloc((size_t)ncell, sizeof(int64_t));
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t k = aff[i];
+        if (type[i] != 0 || k < 0 || k >= ncell) continue;
+        sorted[cnt[k] + fill[k]++] = d2[i];
+    }
+    double *thr = malloc((size_t)(ncell > 0 ? ncell : 1) * sizeof(double));
+    for (int64_t k = 0; k < ncell; ++k) {
+        const int64_t m = cnt[k + 1] - cnt[k];
+        thr[k] = 0.0;
+        if (m == 0) continue;
+        qsort(sorted + cnt[k], (size_t)m, sizeof(double), cmp_double);
+        thr[k] = sorted[cnt[k] + m / 2] * tol * tol;       /* src/cleanup.h:52-54 */
+    }
+    int64_t nd = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t k = aff[i];
+        if (type[i] != 0 || k < 0 || k >= ncell) continue;
+        if (!(d2[i] < thr[k])) { del[i] = 1; ++nd; }       /* keep iff d2 < threshold, :55-58 */
+    }
+    free(d2); free(cnt); free(sorted); free(fill); free(thr);
+    return nd;
+}

@@ -23,6 +23,9 @@
  *   orc_morton_ref        : brute-force bit interleave, golden vectors.
  *   orc_partition/stencil : brute force by definition, golden vectors.
  *   orc_volume            : sphere closed form.
+ *   orc_stray_delete      : hand-computed medians/thresholds (even and odd
+ *                           cells, ties at the threshold), reference golden
+ *                           vectors from delete_lipid itself.
  */
 #ifndef ORBC_ORACLE_H_
 #define ORBC_ORACLE_H_
@@ -140,6 +143,12 @@ int64_t orc_stencil(int64_t ncell, const float *c, int64_t cell, float rmax, int
 int64_t orc_pairs_voronoi_fp64(int64_t n, const double *x, const int32_t *type,
                                const int32_t *aff, int64_t ncell, const float *c,
                                int64_t cap, int64_t *out);
+/* delete_lipid, src/cleanup.h:29-60: per cell the squared distances of its
+ * lipids to the centroid (FP64 against the FP32 centroid) are sorted and a
+ * lipid is deleted (del = 1) unless d2 < tol^2 * dr2[size / 2].  Proteins and
+ * lipids with aff < 0 are kept.  Returns the number deleted. */
+int64_t orc_stray_delete(int64_t n, const double *x, const int32_t *type, const int32_t *aff, int64_t ncell,
+                         const float *c, double tol, uint8_t *del);
 /* Cone-sum volume + push, fixed mode (normals reset, type[j]).
  * src/constrain_volume.h:26-83.  Adds to f, returns volume. */
 double orc_constrain_volume(int64_t n, const double *nrm, const int32_t *type,

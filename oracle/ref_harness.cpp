@@ -24,6 +24,7 @@
 #include "assign_temperature.h"
 
 #include <cstdio>
+#include <map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -277,6 +278,38 @@ void ref_partition(int n, const float *x, int ncell, const float *c, int *aff) {
     for (int i = 0; i < n; ++i) cl.affiliation[i] = -1;
     cl.partition(cont, voronoi);
     for (int i = 0; i < n; ++i) aff[i] = cl.affiliation[i];
+}
+
+// delete_lipid, src/cleanup.h:29-91, on n lipids and fixed centroids: the
+// cells are the reference's own partition and reorder (VCellList::update);
+// aff[i] is the cell of input lipid i and keep[i] whether it survives
+// (survivors are recognised by their positions, which delete_lipid copies
+// unchanged; the inputs must be distinct points).
+void ref_stray(int n, const float *x, int ncell, const float *c, double tol, int *aff, unsigned char *keep) {
+    omp_set_num_threads(1);
+    RTParameter *param = make_param(0.22, 0.01, 0.01, 1);
+    param->stray_tolerance = (float)tol;
+    VoronoiDiagram voronoi(ncell);
+    voronoi.centroids.resize(ncell);
+    for (int k = 0; k < ncell; ++k) voronoi.centroids[k] = vector<real, 3>(c[3 * k], c[3 * k + 1], c[3 * k + 2]);
+    voronoi.tree.build(voronoi.centroids);
+    LipidContainer lipid("lipid");
+    load(lipid, n, x, nullptr, nullptr, nullptr, nullptr, nullptr);
+    lipid.tag2idx.build_map(lipid);
+    VCellList cl;
+    cl.affiliation.resize(n);
+    for (int i = 0; i < n; ++i) cl.affiliation[i] = -1;
+    QuietCout q;
+    cl.update(lipid, voronoi, *param);
+    std::map<std::vector<float>, int> where;
+    for (int i = 0; i < n; ++i) where[{x[3 * i], x[3 * i + 1], x[3 * i + 2]}] = i;
+    for (int k = 0; k < ncell; ++k)
+        for (int j = cl.cell_start[k]; j < cl.cell_start[k + 1]; ++j)
+            aff[where[{lipid.x[j][0], lipid.x[j][1], lipid.x[j][2]}]] = k;
+    delete_lipid(lipid, voronoi, cl, *param);
+    for (int i = 0; i < n; ++i) keep[i] = 0;
+    for (std::size_t j = 0; j < lipid.size(); ++j) keep[where[{lipid.x[j][0], lipid.x[j][1], lipid.x[j][2]}]] = 1;
+    delete param;
 }
 
 // The reference main loop (src/openrbc.cpp:155-256, default build: fused

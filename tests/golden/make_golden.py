@@ -94,6 +94,24 @@ def main():
     aff = R.partition(q, cen)
     np.savez_compressed(os.path.join(HERE, "voronoi.npz"), centroids=cen, queries=q, nearest=near,
                         within_counts=counts, within_flat=flat, partition=aff)
+
+    # stray-lipid cleanup (delete_lipid) on a jittered sheet with lifted lipids
+    sheet = patch(40, 40, seed=11, proteins=False)["x"]
+    lift = rng.choice(len(sheet), 24, replace=False)
+    sheet[lift, 2] += rng.uniform(0.4, 6.0, len(lift))
+    xs = sheet.astype(np.float32)
+    cen = xs[7::14].copy()
+    cen[:, 2] = 0.0
+    aff, keep = R.stray(xs, cen, 2.5)
+    # no lipid within 1e-4 (relative) of its cell's threshold: FP32 and FP64 decide alike
+    d2 = np.sum((xs.astype(np.float64) - cen[aff].astype(np.float64)) ** 2, 1)
+    thr = np.zeros(len(cen))
+    for k in range(len(cen)):
+        m = np.sort(d2[aff == k])
+        thr[k] = m[len(m) // 2] * 6.25 if len(m) else 0.0
+    assert np.min(np.abs(d2 / np.maximum(thr[aff], 1e-300) - 1)) > 1e-4
+    np.savez_compressed(os.path.join(HERE, "stray.npz"), x=xs, centroids=cen, tol=2.5, aff=aff, keep=keep)
+    print("stray cleanup: %d of %d lipids removed" % (int((~keep).sum()), len(keep)))
     print("wrote fixtures to", HERE)
 
 

@@ -406,7 +406,7 @@ def test_run_with_volume_constraint_matches_call_sequence(skin):
 
     def make(target):
         sim = orbc.Simulation(precision=orbc.FP64, seed=11, skin=skin, kBT=0.22, volume_target=target,
-                              volume_strength=0.5)
+                              volume_strength=0.5, freq_cleanup=0)
         orbc.system_to_simulation(sphere, sim, kT=0.22, tilt=0.02, seed=3)
         sim.voronoi_init(0, 4)
         return sim
@@ -433,6 +433,96 @@ def test_run_with_volume_constraint_matches_call_sequence(skin):
     assert np.abs(ca - cb).max() < 1e-4
     if skin < 0.1:
         assert run_sim.observables()["rebuilds"] > 3
+
+
+# ---- stray-lipid cleanup (delete_lipid, src/cleanup.h:29-91) ----------------------------
+def _lifted_sphere(r=12.0, seed=2, n_lift=12, lift=4.0):
+    sphere = orbc.random_sphere(r=r, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    idx = rng.choice(len(sphere["lipid_x"]), n_lift, replace=False)
+    sphere["lipid_x"][idx] *= (r + lift) / r              # radially out of the membrane
+    return sphere, idx
+
+
+@pytest.mark.parametrize("precision", [orbc.FP64, orbc.MIXED])
+def test_stray_cleanup_vs_oracle(precision, oracle):
+    """orbc_delete_stray_lipids removes exactly the lipids the oracle's
+    delete_lipid removes on the device's own cells; the survivors keep their
+    upload order and state; the compacted store evaluates forces as the
+    oracle does on the surviving system."""
+    sphere, lifted = _lifted_sphere()
+    sim = orbc.Simulation(precision=precision, seed=5, freq_cleanup=0)
+    orbc.system_to_simulation(sphere, sim)
+    sim.voronoi_init(0, 4)
+    c, aff, _ = sim.voronoi_get()
+    before = sim.get_state(orbc.LIPID)
+    x0 = before["x"]
+    exp = oracle.stray_delete(x0, np.zeros(len(x0), np.int32), aff, c, 2.5)
+    # most lifted lipids are strays (a lifted lipid that pulled its own cell's
+    # centroid, or shares a cell with another, may stay)
+    assert exp[lifted].sum() >= len(lifted) // 2
+    nd = sim.delete_stray_lipids(2.5)
+    assert nd == exp.sum() and sim.n[orbc.LIPID] == len(x0) - nd
+    obs = sim.observables()
+    assert obs["n_lipid"] == len(x0) - nd and obs["lipids_deleted"] == nd
+    after = sim.get_state(orbc.LIPID)
+    for k in ("x", "v", "n", "o"):
+        assert np.array_equal(after[k], before[k][~exp]), k
+    # forces of the compacted store (lipids only) against the oracle on the survivors
+    sim.compute_pairwise(zero_first=True)
+    f, t = sim.get_forces(orbc.LIPID)
+    xs, ns = x0[~exp], before["n"][~exp]
+    of, ot, _, _ = oracle.forces(xs, ns, np.zeros(len(xs), np.int32))
+    if precision == orbc.FP64:
+        assert eps_f(f, of) < EPS_FP64 and eps_f(t, ot) < EPS_FP64
+    else:
+        # an unrelaxed random sphere (near-overlapping lipids whose pair terms reach thousands
+        # while the sums are tens): FP32 mirror rounding gives up to ~5e-5 here, so this only
+        # checks the compacted store (a wrong neighbour or a stale particle is an O(1) error);
+        # the mixed-precision bound itself is tested on relaxed systems
+        fn = np.linalg.norm(of, axis=1)
+        assert np.max(np.linalg.norm(f - of, axis=1) / np.maximum(fn, np.median(fn))) < 1e-3
+    # nothing left to remove with the cells recomputed
+    sim.voronoi_update()
+    assert sim.delete_stray_lipids(2.5) >= 0
+
+
+def test_run_with_cleanup_matches_call_sequence():
+    """orbc_run removes strays every freq_cleanup steps on Voronoi-update
+    steps, before the update (src/openrbc.cpp:200-202): equal to the per-call
+    sequence delete_stray_lipids, voronoi_update, forces, constrain_volume,
+    integrate.  The run crosses two cleanup steps (0 and 20)."""
+    sphere, lifted = _lifted_sphere(seed=4)
+
+    def make(fc, target):
+        sim = orbc.Simulation(precision=orbc.FP64, seed=11, skin=0.3, kBT=0.22, volume_target=target,
+                              volume_strength=0.5, freq_cleanup=fc)
+        orbc.system_to_simulation(sphere, sim, kT=0.22, tilt=0.02, seed=3)
+        sim.voronoi_init(0, 4)
+        return sim
+
+    call_sim = make(0, 0.0)
+    V0 = 1.02 * call_sim.constrain_volume(1.0, 0.0)
+    run_sim = make(20, V0)
+    nsteps = 30
+    run_sim.run(nsteps)
+    removed = 0
+    for step in range(nsteps):
+        if step % 2 == 0:
+            if step % 20 == 0:
+                removed += call_sim.delete_stray_lipids(2.5)
+            call_sim.voronoi_update()
+        call_sim.compute_pairwise(zero_first=True)
+        call_sim.compute_bonded()
+        call_sim.constrain_volume(V0, 0.5)
+        call_sim.integrate("verlet_langevin")
+    assert removed >= len(lifted) // 2
+    assert run_sim.observables()["lipids_deleted"] == removed
+    a = run_sim.get_state(orbc.LIPID)
+    b = call_sim.get_state(orbc.LIPID)
+    assert a["x"].shape == b["x"].shape
+    for k in ("x", "v", "n"):
+        assert np.abs(a[k] - b[k]).max() < 1e-8 * max(1.0, np.abs(b[k]).max()), k
 
 
 def test_run_volume_needs_cells():
