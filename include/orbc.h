@@ -207,6 +207,32 @@ int orbc_nearest(orbc_ctx *ctx, long long npts, const float *pts, long long nq, 
 int orbc_within(orbc_ctx *ctx, long long npts, const float *pts, long long nq, const float *q, float rmax,
                 int cap, int *counts, int *out);
 
+/* Trajectory and topology files (src/trajectory.h:26-150, src/topology.h:23-52).
+ * orbc_save_frame writes (append != 0: appends) one frame in the reference's
+ * binary layout: "FRAMEBEG" step (int32), "NATOM" count (uint64),
+ * "IDENTITY" tag and type (int32 each), then the sections dump_field selects
+ * with the reference's DumpField bits (1 POSITION, 2 ROTATION, 4 VORONOI,
+ * 8 VELOCITY, 16 FORCE; FP32 vectors, int32 cells), "FRAMEEND".  Lipids
+ * first, then proteins, each in upload order.
+ * orbc_read_frame reads frame `index` (0-based): *step, *natom, *fields (the
+ * DumpField bits present); array arguments may be NULL (call once with NULLs
+ * to size them; tag/type hold natom entries, vectors 3 natom).
+ * orbc_save_topology writes the reference's LAMMPS data file (proteins, then
+ * lipids; bonds by tag). */
+int orbc_save_frame(orbc_ctx *ctx, const char *path, int dump_field, int append);
+int orbc_read_frame(const char *path, long long index, long long *step, long long *natom, int *fields,
+                    unsigned *tag, int *type, float *x, float *v, float *n, int *aff, float *f);
+int orbc_save_topology(orbc_ctx *ctx, const char *path);
+/* Restart files (no reference counterpart; row f2's resume): the complete
+ * FP64 state (configuration with the current Nose-Hoover friction and
+ * resolved volume target, step counter, both species with x, v, n, o and the
+ * force accumulators, ensemble members, bonds, Voronoi centroids and cells).
+ * orbc_load_restart replaces the context's configuration (its device must
+ * match) and state; a run continued from it follows the uninterrupted run
+ * (same noise: it is keyed by particle id and step). */
+int orbc_save_restart(orbc_ctx *ctx, const char *path);
+int orbc_load_restart(orbc_ctx *ctx, const char *path);
+
 int orbc_synchronize(orbc_ctx *ctx);
 
 /* Kernel timing of orbc_run with HIP events on the context's stream:

@@ -127,6 +127,12 @@ SIGNATURES = [
     ("orbc_nearest", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, _i]),
     ("orbc_within", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, C.c_float, C.c_int, _i, _i]),
     ("orbc_synchronize", C.c_int, [_vp]),
+    ("orbc_save_frame", C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int]),
+    ("orbc_read_frame", C.c_int, [C.c_char_p, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                  _i, _u, _i, _f, _f, _f, _i, _f]),
+    ("orbc_save_topology", C.c_int, [_vp, C.c_char_p]),
+    ("orbc_save_restart", C.c_int, [_vp, C.c_char_p]),
+    ("orbc_load_restart", C.c_int, [_vp, C.c_char_p]),
     ("orbc_set_timing", C.c_int, [_vp, C.c_int]),
     ("orbc_get_timing", C.c_int, [_vp, _d, C.POINTER(C.c_longlong), _d, C.POINTER(C.c_longlong)]),
     ("orbc_init_rbc", C.c_int, [_d, C.c_longlong, _i, C.c_longlong, _i, C.c_longlong, C.c_double, C.c_ulonglong,
@@ -135,6 +141,37 @@ SIGNATURES = [
 ]
 
 _LIB = None
+
+
+# DumpField bits of the trajectory format (src/runtime_parameter.h:27-33)
+POSITION, ROTATION, VORONOI, VELOCITY, FORCE = 1, 2, 4, 8, 16
+
+
+def read_frame(path: str, index: int = 0) -> dict:
+    """read_frame (src/trajectory.h:107-150): frame `index` of a trajectory
+    file as arrays (tag, type, and the fields present: x, n, aff, v, f)."""
+    L = load_library()
+    step, natom, fields = C.c_longlong(), C.c_longlong(), C.c_int()
+    p = os.fsencode(path)
+    rc = L.orbc_read_frame(p, int(index), C.byref(step), C.byref(natom), C.byref(fields), None, None, None, None,
+                           None, None, None)
+    if rc != 0:
+        raise OrbcError(rc, f"cannot read frame {index} of {path}")
+    m = natom.value
+    tag = np.zeros(m, np.uint32); typ = np.zeros(m, np.int32); aff = np.zeros(m, np.int32)
+    vec = {k: np.zeros((m, 3), np.float32) for k in ("x", "v", "n", "f")}
+    rc = L.orbc_read_frame(p, int(index), C.byref(step), C.byref(natom), C.byref(fields), _ptr(tag, _u),
+                           _ptr(typ, _i), _ptr(vec["x"], _f), _ptr(vec["v"], _f), _ptr(vec["n"], _f),
+                           _ptr(aff, _i), _ptr(vec["f"], _f))
+    if rc != 0:
+        raise OrbcError(rc, f"cannot read frame {index} of {path}")
+    out = dict(step=step.value, natom=m, fields=fields.value, tag=tag, type=typ)
+    for bit, key in ((POSITION, "x"), (VELOCITY, "v"), (ROTATION, "n"), (FORCE, "f")):
+        if fields.value & bit:
+            out[key] = vec[key]
+    if fields.value & VORONOI:
+        out["aff"] = aff
+    return out
 
 
 def load_library(path: str = LIB_PATH):
@@ -348,6 +385,29 @@ class Simulation:
     def synchronize(self):
         self._check(self.L.orbc_synchronize(self.h))
 
+    # -- trajectory / topology files (src/trajectory.h, src/topology.h) ------------
+    def save_frame(self, path: str, dump_field: int = POSITION | ROTATION | VORONOI, append: bool = True):
+        """save_frame: one frame of the reference's binary trajectory format."""
+        self._check(self.L.orbc_save_frame(self.h, os.fsencode(path), int(dump_field), int(append)))
+
+    def save_topology(self, path: str):
+        """save_topology: the reference's LAMMPS data file."""
+        self._check(self.L.orbc_save_topology(self.h, os.fsencode(path)))
+
+    def save_restart(self, path: str):
+        """The complete FP64 state, for orbc_load_restart."""
+        self._check(self.L.orbc_save_restart(self.h, os.fsencode(path)))
+
+    def load_restart(self, path: str):
+        """Replace this context's configuration and state by a restart file."""
+        try:
+            self._check(self.L.orbc_load_restart(self.h, os.fsencode(path)))
+        finally:
+            self._sync_counts()
+        cfg = Config()
+        if self.L.orbc_get_config(self.h, C.byref(cfg)) == 0:
+            self.cfg = cfg
+
     def set_timing(self, on: bool = True):
         self._check(self.L.orbc_set_timing(self.h, int(on)))
 
@@ -433,5 +493,6 @@ from .systems import (evans_fung, icosphere, init_rbc, load_example_large, plana
 __all__ = [
     "Config", "Container", "Simulation", "OrbcError", "default_config", "load_library",
     "compute_pairwise_fused", "compute_bonded", "integrate", "constrain_volume", "compute_temperature",
+    "read_frame", "POSITION", "ROTATION", "VORONOI", "VELOCITY", "FORCE",
     "evans_fung", "icosphere", "init_rbc", "load_example_large", "planar_membrane", "planar_patch", "random_sphere", "system_to_simulation", "vesicle_ensemble",
 ]

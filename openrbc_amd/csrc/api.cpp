@@ -165,10 +165,12 @@ void upload_bonds(orbc_ctx *c) {
     // (partner slot, type) so that the summation order is fixed
     std::unordered_map<uint32_t, uint32_t> id2slot;
     id2slot.reserve((size_t)c->np * 2 + 1);
-    for (long long u = 0; u < c->np; ++u) id2slot[c->sp[1].id[u]] = (uint32_t)(c->nl + u);
+    for (long long u = 0; u < c->np; ++u) id2slot[c->sp[1].id[u]] = (uint32_t)(c->pslot0 + u);
     for (long long u = 0; u < c->nl; ++u) id2slot[c->sp[0].id[u]] = (uint32_t)u;
     const long long nb = (long long)c->b_type.size();
-    std::vector<std::vector<std::pair<uint32_t, uint8_t>>> lists((size_t)c->n);
+    // slots: lipids [0, nl), proteins [pslot0, pslot0 + np) (pslot0 > nl after a stray cleanup)
+    const long long nslot = c->pslot0 + c->np;
+    std::vector<std::vector<std::pair<uint32_t, uint8_t>>> lists((size_t)nslot);
     for (long long k = 0; k < nb; ++k) {
         auto a = id2slot.find(c->b_i[k]), b = id2slot.find(c->b_j[k]);
         if (a == id2slot.end() || b == id2slot.end()) throw std::runtime_error("bond refers to an unknown particle id");
@@ -176,19 +178,19 @@ void upload_bonds(orbc_ctx *c) {
         lists[a->second].push_back({b->second, (uint8_t)c->b_type[k]});
         lists[b->second].push_back({a->second, (uint8_t)c->b_type[k]});
     }
-    std::vector<uint32_t> start((size_t)c->n + 1, 0), partner;
+    std::vector<uint32_t> start((size_t)nslot + 1, 0), partner;
     std::vector<uint8_t> bt;
     partner.reserve((size_t)(2 * nb));
     bt.reserve((size_t)(2 * nb));
     int kb = 0;
-    for (long long s = 0; s < c->n; ++s) {
+    for (long long s = 0; s < nslot; ++s) {
         auto &L = lists[s];
         std::sort(L.begin(), L.end());
         start[s] = (uint32_t)partner.size();
         for (auto &p : L) { partner.push_back(p.first); bt.push_back(p.second); }
         kb = std::max<int>(kb, (int)L.size());
     }
-    start[c->n] = (uint32_t)partner.size();
+    start[nslot] = (uint32_t)partner.size();
     if (kb > 255) throw std::runtime_error("more than 255 bonds on one particle");
     c->mem.release(c->bstart); c->mem.release(c->bpartner); c->mem.release(c->btype_d); c->mem.release(c->bnd);
     c->bstart = c->mem.alloc<uint32_t>(start.size());
@@ -331,7 +333,7 @@ void set_cutoffs(orbc_ctx *c, double cs) {
 
 // gathers every per-particle array of the store through perm (n_out
 // entries: perm[i] is the old index of the new particle i)
-static void permute_store(orbc_ctx *c, const uint32_t *perm, long long n_out) {
+static void permute_store(orbc_ctx *c, const uint32_t *perm, long long n_out, bool with_ft = true) {
     hipStream_t s = c->stream;
     PermuteArgs p{};
     p.n = n_out; p.perm = perm;
@@ -340,7 +342,10 @@ static void permute_store(orbc_ctx *c, const uint32_t *perm, long long n_out) {
     p.src3[1] = c->nrm[c->cur]; p.dst3[1] = c->nrm[nx];
     p.src3[2] = c->v; p.dst3[2] = c->v_alt;
     p.src3[3] = c->o; p.dst3[3] = c->o_alt;
-    if (c->mixed()) {
+    if (!with_ft) {
+        // the force accumulators hold nothing the caller can see (inside orbc_run)
+        p.n3 = 4; p.nf3 = 0;
+    } else if (c->mixed()) {
         p.n3 = 4;
         p.srcf3[0] = c->f32; p.dstf3[0] = c->f32_alt;
         p.srcf3[1] = c->t32; p.dstf3[1] = c->t32_alt;
@@ -358,8 +363,10 @@ static void permute_store(orbc_ctx *c, const uint32_t *perm, long long n_out) {
     launch_permute(p, s);
     c->cur = nx;
     std::swap(c->v, c->v_alt); std::swap(c->o, c->o_alt);
-    std::swap(c->f32, c->f32_alt); std::swap(c->t32, c->t32_alt);
-    std::swap(c->f64, c->f64_alt); std::swap(c->t64, c->t64_alt);
+    if (with_ft) {
+        std::swap(c->f32, c->f32_alt); std::swap(c->t32, c->t32_alt);
+        std::swap(c->f64, c->f64_alt); std::swap(c->t64, c->t64_alt);
+    }
     std::swap(c->type, c->type_alt); std::swap(c->id, c->id_alt); std::swap(c->slot, c->slot_alt);
     if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
     if (c->grp) std::swap(c->grp, c->grp_alt);
@@ -468,7 +475,7 @@ void rebuild(orbc_ctx *c) {
                      &alt);
     const uint32_t *perm = alt ? c->vals_alt : c->vals;
     // 5. permute the store
-    permute_store(c, perm, n);
+    permute_store(c, perm, n, !c->ft_scratch);
     launch_inverse(c->slot, c->inv, n, s);
     // 6. mirrors, anchors, build-time offsets
     launch_refresh(n, c->x[c->cur], c->nrm[c->cur], c->type, ag, c->pq[c->cur], c->rb, c->anc, s);
@@ -606,20 +613,25 @@ void orbc_config_default(orbc_config *cfg) {
     cfg->block_size = 0;
 }
 
+static bool config_valid(const orbc_config &cfg_) {
+    if (cfg_.dt <= 0 || cfg_.skin <= 0 || cfg_.kBT < 0 || cfg_.eta < 0) return false;
+    if (cfg_.precision != ORBC_FP64 && cfg_.precision != ORBC_MIXED) return false;
+    for (int d = 0; d < 3; ++d)
+        if (cfg_.periodic[d] && !(cfg_.box_hi[d] > cfg_.box_lo[d])) return false;
+    // src/runtime_parameter.h:118-120 (reported, not exit(0))
+    if (cfg_.freq_voronoi <= 0 || cfg_.freq_sort_ctrd % cfg_.freq_voronoi || cfg_.freq_sort_bond % cfg_.freq_voronoi ||
+        cfg_.freq_cleanup % cfg_.freq_voronoi)
+        return false;
+    return true;
+}
+
 int orbc_create(const orbc_config *cfg, orbc_ctx **out) {
     if (!out) return ORBC_ERR_INVALID;
     *out = nullptr;
     orbc_config c0;
     orbc_config_default(&c0);
     const orbc_config &cfg_ = cfg ? *cfg : c0;
-    if (cfg_.dt <= 0 || cfg_.skin <= 0 || cfg_.kBT < 0 || cfg_.eta < 0) return ORBC_ERR_INVALID;
-    if (cfg_.precision != ORBC_FP64 && cfg_.precision != ORBC_MIXED) return ORBC_ERR_INVALID;
-    for (int d = 0; d < 3; ++d)
-        if (cfg_.periodic[d] && !(cfg_.box_hi[d] > cfg_.box_lo[d])) return ORBC_ERR_INVALID;
-    // src/runtime_parameter.h:118-120 (reported, not exit(0))
-    if (cfg_.freq_voronoi <= 0 || cfg_.freq_sort_ctrd % cfg_.freq_voronoi || cfg_.freq_sort_bond % cfg_.freq_voronoi ||
-        cfg_.freq_cleanup % cfg_.freq_voronoi)
-        return ORBC_ERR_INVALID;
+    if (!config_valid(cfg_)) return ORBC_ERR_INVALID;
     orbc_ctx *c = new orbc_ctx();
     c->cfg = cfg_;
     hipError_t e = hipSetDevice(cfg_.device);
@@ -640,6 +652,46 @@ int orbc_create(const orbc_config *cfg, orbc_ctx **out) {
     *out = c;
     return ORBC_OK;
 }
+
+}  // extern "C"
+
+namespace orbc {
+
+// a new configuration for an existing context (restart files): the device
+// state is dropped and the run counters reset; the caller uploads the state
+int orbc_set_config_internal(orbc_ctx *c, const orbc_config &cfg) {
+    if (!config_valid(cfg) || cfg.device != c->cfg.device) return fail(c, ORBC_ERR_INVALID, "invalid configuration");
+    if (c->live) {
+        voronoi_free(c);
+        c->mem.free_all();
+        c->live = false;
+    }
+    c->cfg = cfg;
+    c->group_kbt.clear();
+    for (auto &S : c->sp) S.group.clear();
+    c->step = 0; c->nh_started = false; c->deleted = 0; c->last_cleanup = -1;
+    c->vol_target_set = false; c->list_life = 0.0; c->since_build = 0;
+    return ORBC_OK;
+}
+
+// cells of every particle, given per species in upload order
+void voronoi_set_affiliation(orbc_ctx *c, const int *aff_lipid, const int *aff_protein) {
+    std::vector<uint32_t> hs(c->n);
+    ORBC_HIP_CHECK(hipMemcpyAsync(hs.data(), c->slot, 4 * c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    std::vector<int32_t> hv(c->n);
+    for (long long i = 0; i < c->n; ++i) {
+        const long long s = hs[i];
+        hv[i] = s < c->pslot0 ? aff_lipid[s] : aff_protein[s - c->pslot0];
+    }
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->voro.aff, hv.data(), 4 * c->n, hipMemcpyHostToDevice, c->stream));
+    c->voro.aff_valid = true;
+    sync(c);
+}
+
+}  // namespace orbc
+
+extern "C" {
 
 int orbc_destroy(orbc_ctx *c) {
     if (!c) return ORBC_ERR_INVALID;
@@ -986,6 +1038,17 @@ static void volume_begin(orbc_ctx *c) {
 
 static void run_langevin(orbc_ctx *c, long long nsteps) {
     long long remaining = nsteps;
+    // the fused step keeps forces in registers: rebuilds inside the run leave
+    // the accumulators alone, and they are cleared when it ends (as
+    // integrate(verlet_langevin) leaves f = t = 0, src/integrate_langevin.h:148)
+    struct Scratch {
+        orbc_ctx *c;
+        ~Scratch() {
+            c->ft_scratch = false;
+            try { clear_ft(c, false); } catch (...) {}      // an error of the run is already on its way out
+        }
+    } scratch{c};
+    c->ft_scratch = true;
     const bool vol = volume_on(c);
     const bool cells = cells_in_run(c);
     const long long cperiod = cleanup_period(c);

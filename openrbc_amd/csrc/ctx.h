@@ -68,6 +68,7 @@ struct orbc_ctx {
     long long pslot0 = 0;         // first protein slot (upload order: lipids, then proteins)
     long long last_cleanup = -1;  // step of the last stray-lipid cleanup inside orbc_run
     long long deleted = 0;        // lipids removed by the cleanup so far
+    bool ft_scratch = false;      // the force accumulators are scratch (inside orbc_run's Langevin loop)
     double *x[2] = {nullptr, nullptr}, *nrm[2] = {nullptr, nullptr};
     int cur = 0;
     double *v = nullptr, *o = nullptr, *v_alt = nullptr, *o_alt = nullptr;
@@ -171,6 +172,9 @@ void voronoi_update_dev(orbc_ctx *c);
 // the strays from the store and rebuilds.  Returns the number removed.
 long long delete_stray_lipids(orbc_ctx *c, double tolerance);
 void compact_store(orbc_ctx *c, const uint32_t *del, long long ndel);
+// restart files (io.cpp): replace the configuration; restore cells by upload order
+int orbc_set_config_internal(orbc_ctx *c, const orbc_config &cfg);
+void voronoi_set_affiliation(orbc_ctx *c, const int *aff_lipid, const int *aff_protein);
 double volume_push(orbc_ctx *c, double target, double strength, double *f_ext, bool to_f, bool read_back);
 void volume_eval(orbc_ctx *c, double target, double strength);
 }  // namespace orbc

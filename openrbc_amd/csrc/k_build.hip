@@ -377,6 +377,13 @@ __global__ __launch_bounds__(BLOCK) void k_cell_compact(const uint32_t *flag, co
 // and the list cutoff comes from the particle type's row held in registers.
 // Particle types whose cutoff + skin exceeds one cell (band-3 / actin with the
 // sigma-3.4 LJ) get the outer shells appended by k_nlist_far afterwards.
+// candidate rounds unrolled in the list build (LDS reads of the next round
+// issued before this round's ballot)
+#ifndef ORBC_NL_UNROLL
+#define ORBC_NL_UNROLL 1
+#endif
+#define ORBC_STR(x) #x
+#define ORBC_PRAGMA(x) _Pragma(ORBC_STR(x))
 __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *occ, long long nocc) {
     __shared__ float4 s_x[BLOCK / 64][NL_CAP];
     __shared__ uint32_t s_pref[BLOCK / 64][28];
@@ -468,6 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_nlist(NlistArgs a, const uint32_t *oc
                 s_x[w][t] = make_float4(pj.x, pj.y, pj.z, __uint_as_float(j | ((uint32_t)a.type[j] << 29)));
             }
             wave_sync();
+ORBC_PRAGMA(unroll ORBC_NL_UNROLL)
             for (uint32_t t0 = 0; t0 < m; t0 += S) {
                 const uint32_t t = t0 + stripe;
                 const bool valid = act && t < m;

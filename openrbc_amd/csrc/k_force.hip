@@ -416,6 +416,22 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
     }
 }
 
+// FP64: one gathered neighbour against owner i (minimum image on periodic axes)
+template <bool ENERGY>
+__device__ __forceinline__ void pair_fp64(const Anchor &ag, bool periodic, double xi, double yi, double zi, double nix,
+                                          double niy, double niz, int ti, const double (&xj)[3], const double (&nj)[3],
+                                          int tj, const FFS<double> &t, Acc<double> &acc) {
+    double dx = sub_rn(xi, xj[0]);
+    double dy = sub_rn(yi, xj[1]);
+    double dz = sub_rn(zi, xj[2]);
+    if (periodic) min_image(ag, dx, dy, dz);
+    const double r2 = r2_rn(dx, dy, dz);
+    pair_ij<double, ENERGY>(t, ti, tj, dx, dy, dz, dx, dy, dz, r2, nix, niy, niz, nj[0], nj[1], nj[2], acc);
+}
+
+#ifndef ORBC_PIPE64
+#define ORBC_PIPE64 0
+#endif
 template <bool ENERGY>
 __device__ __forceinline__ void range_fp64(long long i, int kb, int ke, long long stride, const double *x,
                                            const double *nrm, const uint8_t *type, const uint32_t *nb,
@@ -425,30 +441,66 @@ __device__ __forceinline__ void range_fp64(long long i, int kb, int ke, long lon
     long long jn[UNROLL64];
 #pragma unroll
     for (int u = 0; u < UNROLL64; ++u) jn[u] = (kb + u < ke) ? (long long)nb[(long long)(kb + u) * stride] : i;
-    for (int k0 = kb; k0 < ke; k0 += UNROLL64) {
-        long long jj[UNROLL64];
+    if constexpr (ORBC_PIPE64) {
+        // gathers of the next group in flight while this group is evaluated
+        double xa[UNROLL64][3], na[UNROLL64][3];
+        int ta[UNROLL64];
 #pragma unroll
         for (int u = 0; u < UNROLL64; ++u) {
-            jj[u] = jn[u];
-            const int kn = k0 + UNROLL64 + u;
+            for (int d = 0; d < 3; ++d) { xa[u][d] = x[3 * jn[u] + d]; na[u][d] = nrm[3 * jn[u] + d]; }
+            ta[u] = type[jn[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL64; ++u) {
+            const int kn = kb + UNROLL64 + u;
             jn[u] = (kn < ke) ? (long long)nb[(long long)kn * stride] : i;
         }
-        double xj[UNROLL64][3], nj[UNROLL64][3];
-        int tj[UNROLL64];
+        for (int k0 = kb; k0 < ke; k0 += UNROLL64) {
+            double xb[UNROLL64][3], nbv[UNROLL64][3];
+            int tb[UNROLL64];
+            const bool more = k0 + UNROLL64 < ke;
+            if (more) {
 #pragma unroll
-        for (int u = 0; u < UNROLL64; ++u) {
-            for (int d = 0; d < 3; ++d) { xj[u][d] = x[3 * jj[u] + d]; nj[u][d] = nrm[3 * jj[u] + d]; }
-            tj[u] = type[jj[u]];
+                for (int u = 0; u < UNROLL64; ++u) {
+                    for (int d = 0; d < 3; ++d) { xb[u][d] = x[3 * jn[u] + d]; nbv[u][d] = nrm[3 * jn[u] + d]; }
+                    tb[u] = type[jn[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL64; ++u) {
+                    const int kn = k0 + 2 * UNROLL64 + u;
+                    jn[u] = (kn < ke) ? (long long)nb[(long long)kn * stride] : i;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL64; ++u)
+                pair_fp64<ENERGY>(ag, periodic, xi, yi, zi, nix, niy, niz, ti, xa[u], na[u], ta[u], t, acc);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < UNROLL64; ++u) {
+                    for (int d = 0; d < 3; ++d) { xa[u][d] = xb[u][d]; na[u][d] = nbv[u][d]; }
+                    ta[u] = tb[u];
+                }
+            }
         }
+    } else {
+        for (int k0 = kb; k0 < ke; k0 += UNROLL64) {
+            long long jj[UNROLL64];
 #pragma unroll
-        for (int u = 0; u < UNROLL64; ++u) {
-            double dx = sub_rn(xi, xj[u][0]);
-            double dy = sub_rn(yi, xj[u][1]);
-            double dz = sub_rn(zi, xj[u][2]);
-            if (periodic) min_image(ag, dx, dy, dz);
-            const double r2 = r2_rn(dx, dy, dz);
-            pair_ij<double, ENERGY>(t, ti, tj[u], dx, dy, dz, dx, dy, dz, r2, nix, niy, niz, nj[u][0], nj[u][1],
-                                    nj[u][2], acc);
+            for (int u = 0; u < UNROLL64; ++u) {
+                jj[u] = jn[u];
+                const int kn = k0 + UNROLL64 + u;
+                jn[u] = (kn < ke) ? (long long)nb[(long long)kn * stride] : i;
+            }
+            double xj[UNROLL64][3], nj[UNROLL64][3];
+            int tj[UNROLL64];
+#pragma unroll
+            for (int u = 0; u < UNROLL64; ++u) {
+                for (int d = 0; d < 3; ++d) { xj[u][d] = x[3 * jj[u] + d]; nj[u][d] = nrm[3 * jj[u] + d]; }
+                tj[u] = type[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL64; ++u)
+                pair_fp64<ENERGY>(ag, periodic, xi, yi, zi, nix, niy, niz, ti, xj[u], nj[u], tj[u], t, acc);
         }
     }
 }

@@ -310,6 +310,10 @@ class Reference:
         L.ref_kdtree_within.argtypes = [C.c_int, _f, C.c_int, _f, C.c_float, C.c_int, _i32, _i32]
         L.ref_partition.argtypes = [C.c_int, _f, C.c_int, _f, _i32]
         L.ref_stray.argtypes = [C.c_int, _f, C.c_int, _f, C.c_double, _i32, _u8]
+        L.ref_save_frame.argtypes = [C.c_char_p, C.c_int, _f, _f, _f, _i32, C.c_int, C.c_int, _f, _f, _f, _i32,
+                                     _i32, _i32, C.c_int, C.c_int, C.c_int]
+        L.ref_save_topology.argtypes = [C.c_char_p, C.c_int, _f, C.c_int, C.c_int, _f, _i32, _i32, C.c_int,
+                                        _i32, _i32, _i32]
         L.ref_run.restype = C.c_double
         L.ref_run.argtypes = [C.c_int, _f, _f, C.c_int, _f, _f, _i32, _i32, C.c_int, _i32, _i32, _i32,
                               C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -389,6 +393,23 @@ class Reference:
         aff = np.zeros(len(x), np.int32)
         self.L.ref_partition(len(x), _p(x, _f), len(c), _p(c, _f), _p(aff, _i32))
         return aff
+
+    def save_frame(self, path, lx, lv, ln, laff, lbase, px, pv, pn, paff, ptype, ptag, nstep, dump_field,
+                   append=False):
+        """save_frame of containers built from the arrays (FP32)."""
+        a = [_c(v, np.float32) for v in (lx, lv, ln, px, pv, pn)]
+        la, pa = _c(laff, np.int32), _c(paff, np.int32)
+        pt, pg = _c(ptype, np.int32), _c(ptag, np.int32)
+        self.L.ref_save_frame(path.encode(), len(a[0]), _p(a[0], _f), _p(a[1], _f), _p(a[2], _f), _p(la, _i32),
+                              int(lbase), len(a[3]), _p(a[3], _f), _p(a[4], _f), _p(a[5], _f), _p(pa, _i32),
+                              _p(pt, _i32), _p(pg, _i32), int(nstep), int(dump_field), int(append))
+
+    def save_topology(self, path, lx, lbase, px, ptype, ptag, bt, bi, bj):
+        lx, px = _c(lx, np.float32), _c(px, np.float32)
+        pt, pg = _c(ptype, np.int32), _c(ptag, np.int32)
+        bt, bi, bj = _c(bt, np.int32), _c(bi, np.int32), _c(bj, np.int32)
+        self.L.ref_save_topology(path.encode(), len(lx), _p(lx, _f), int(lbase), len(px), _p(px, _f),
+                                 _p(pt, _i32), _p(pg, _i32), len(bt), _p(bt, _i32), _p(bi, _i32), _p(bj, _i32))
 
     def stray(self, x, centroids, tol):
         """delete_lipid on lipids x with fixed centroids: (cell of each lipid, survives)."""

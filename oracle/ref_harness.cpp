@@ -22,8 +22,11 @@
 #include "kdtree.h"
 #include "cleanup.h"
 #include "assign_temperature.h"
+#include "trajectory.h"
+#include "topology.h"
 
 #include <cstdio>
+#include <fstream>
 #include <map>
 #include <sstream>
 #include <string>
@@ -309,6 +312,48 @@ void ref_stray(int n, const float *x, int ncell, const float *c, double tol, int
     delete_lipid(lipid, voronoi, cl, *param);
     for (int i = 0; i < n; ++i) keep[i] = 0;
     for (std::size_t j = 0; j < lipid.size(); ++j) keep[where[{lipid.x[j][0], lipid.x[j][1], lipid.x[j][2]}]] = 1;
+    delete param;
+}
+
+// save_frame (src/trajectory.h:61-105) and save_topology (src/topology.h:23-52)
+// of containers loaded from the arrays (lipid tags are base + i, as the
+// reference's LipidContainer numbers them).
+void ref_save_frame(const char *path, int nl, const float *lx, const float *lv, const float *ln, const int *laff,
+                    int lbase, int np, const float *px, const float *pv, const float *pn, const int *paff,
+                    const int *ptype, const int *ptag, int nstep, int dump_field, int append) {
+    RTParameter *param = make_param(0.22, 0.01, 0.01, 1);
+    param->nstep = nstep;
+    param->dump_field = dump_field;
+    LipidContainer lipid("lipid");
+    ProteContainer protein("protein");
+    load(lipid, nl, lx, lv, ln, nullptr, nullptr, nullptr);
+    load(protein, np, px, pv, pn, nullptr, nullptr, nullptr);
+    load_protein_meta(protein, np, ptype, ptag);
+    lipid.tag.set_base(lbase);
+    VCellList cl, cp;
+    cl.affiliation.resize(nl);
+    cp.affiliation.resize(np);
+    for (int i = 0; i < nl; ++i) cl.affiliation[i] = laff ? laff[i] : -1;
+    for (int i = 0; i < np; ++i) cp.affiliation[i] = paff ? paff[i] : -1;
+    {
+        std::ofstream f(path, append ? (std::ios::binary | std::ios::app) : std::ios::binary);
+        save_frame(f, lipid, protein, cl, cp, *param);
+    }
+    delete param;
+}
+
+void ref_save_topology(const char *path, int nl, const float *lx, int lbase, int np, const float *px,
+                       const int *ptype, const int *ptag, int nb, const int *bt, const int *bi, const int *bj) {
+    RTParameter *param = make_param(0.22, 0.01, 0.01, 1);
+    LipidContainer lipid("lipid");
+    ProteContainer protein("protein");
+    load(lipid, nl, lx, nullptr, nullptr, nullptr, nullptr, nullptr);
+    load(protein, np, px, nullptr, nullptr, nullptr, nullptr, nullptr);
+    load_protein_meta(protein, np, ptype, ptag);
+    lipid.tag.set_base(lbase);
+    protein.bonds.resize(0);
+    for (int k = 0; k < nb; ++k) protein.bonds.emplace_back(bt[k], bi[k], bj[k]);
+    save_topology(std::ofstream(path), *param, protein, lipid);
     delete param;
 }
 

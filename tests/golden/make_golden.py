@@ -112,6 +112,33 @@ def main():
     assert np.min(np.abs(d2 / np.maximum(thr[aff], 1e-300) - 1)) > 1e-4
     np.savez_compressed(os.path.join(HERE, "stray.npz"), x=xs, centroids=cen, tol=2.5, aff=aff, keep=keep)
     print("stray cleanup: %d of %d lipids removed" % (int((~keep).sum()), len(keep)))
+
+    # trajectory frame (all five fields) and LAMMPS topology written by the reference
+    import tempfile
+    s = patch(12, 12, seed=2)
+    lip, prot = split(s)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    lx, ln, lv = f32(s["x"][lip]), f32(s["n"][lip]), f32(s["v"][lip])
+    px, pn, pv = f32(s["x"][prot]), f32(s["n"][prot]), f32(s["v"][prot])
+    ptype = s["type"][prot].astype(np.int32)
+    ptag = s["id"][prot].astype(np.int32)
+    lbase = int(len(prot) + 1)
+    assert np.array_equal(s["id"][lip], np.arange(lbase, lbase + len(lip)))
+    cen = lx[3::14].copy()
+    laff, paff = R.partition(lx, cen), R.partition(px, cen)
+    bt, bi, bj = s["bonds"]
+    bt = bt.astype(np.int32); bit = s["id"][bi].astype(np.int32); bjt = s["id"][bj].astype(np.int32)
+    with tempfile.TemporaryDirectory() as tmp:
+        fp, tp = os.path.join(tmp, "f.orbc"), os.path.join(tmp, "t.data")
+        R.save_frame(fp, lx, lv, ln, laff, lbase, px, pv, pn, paff, ptype, ptag, 0, 31)
+        R.save_frame(fp, lx, lv, ln, laff, lbase, px, pv, pn, paff, ptype, ptag, 5, 7, append=True)
+        R.save_topology(tp, lx, lbase, px, ptype, ptag, bt, bit, bjt)
+        frame = np.frombuffer(open(fp, "rb").read(), np.uint8)
+        topo = np.frombuffer(open(tp, "rb").read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "frame.npz"), lx=lx, ln=ln, lv=lv, px=px, pn=pn, pv=pv, ptype=ptype,
+                        ptag=ptag, lbase=lbase, centroids=cen, laff=laff, paff=paff, bt=bt, bi=bit, bj=bjt,
+                        frame=frame, topo=topo)
+    print("frame fixture: %d bytes, topology %d bytes" % (len(frame), len(topo)))
     print("wrote fixtures to", HERE)
 
 

@@ -525,6 +525,89 @@ def test_run_with_cleanup_matches_call_sequence():
         assert np.abs(a[k] - b[k]).max() < 1e-8 * max(1.0, np.abs(b[k]).max()), k
 
 
+# ---- trajectory / topology files: byte for byte the reference's writers ----------------
+def test_save_frame_and_topology_match_reference(tmp_path):
+    """orbc_save_frame / orbc_save_topology on the state the reference's
+    save_frame / save_topology wrote in tests/golden/frame.npz: the same
+    bytes (lipids then proteins; FP32 fields; cells from the same centroids)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "frame.npz"))
+    nl = len(g["lx"])
+    f64 = lambda a: np.asarray(a, np.float64)
+    sim = orbc.Simulation(precision=orbc.FP64)
+    lids = np.arange(int(g["lbase"]), int(g["lbase"]) + nl, dtype=np.uint32)
+    sim.set_state(orbc.LIPID, f64(g["lx"]), f64(g["ln"]), lids, v=f64(g["lv"]))
+    sim.set_state(orbc.PROTEIN, f64(g["px"]), f64(g["pn"]), g["ptag"].astype(np.uint32), v=f64(g["pv"]),
+                  type=g["ptype"])
+    sim.set_bonds(g["bt"], g["bi"].astype(np.uint32), g["bj"].astype(np.uint32))
+    sim.voronoi_set_centroids(g["centroids"])
+    for sp, m in ((orbc.LIPID, nl), (orbc.PROTEIN, len(g["px"]))):
+        sim.set_forces(sp, np.zeros((m, 3)), np.zeros((m, 3)))
+    fp = tmp_path / "f.orbc"
+    sim.save_frame(str(fp), 31, append=False)
+    ref = g["frame"].tobytes()
+    end = ref.index(b"FRAMEEND") + 8                 # the first of the two reference frames (step 0)
+    assert fp.read_bytes() == ref[:end]
+    tp = tmp_path / "t.data"
+    sim.save_topology(str(tp))
+    assert tp.read_bytes() == g["topo"].tobytes()
+    back = orbc.read_frame(str(fp), 0)
+    assert np.array_equal(back["x"], np.concatenate([g["lx"], g["px"]]))
+
+
+# ---- restart files (resume; no reference counterpart) -----------------------------------
+@pytest.mark.parametrize("thermostat", [orbc.RUN_LANGEVIN, orbc.RUN_NOSE_HOOVER])
+def test_restart_resumes_the_run(system, thermostat, tmp_path):
+    """30 steps == 15 steps, save_restart, load_restart into a fresh context, 15
+    steps.  Same noise (keyed by id and step) and the same Nose-Hoover state;
+    the neighbour order after the reload may differ, so FP64 rounding only."""
+    kw = dict(seed=9, kBT=0.22, thermostat=thermostat, skin=0.3, zeta=0.05, Q=7.0)
+    a, _, _ = make_sim(system, orbc.FP64, **kw)
+    a.run(30)
+    b, _, _ = make_sim(system, orbc.FP64, **kw)
+    b.run(15)
+    path = str(tmp_path / "r.rst")
+    b.save_restart(path)
+    c = orbc.Simulation()
+    c.load_restart(path)
+    assert c.cfg.thermostat == thermostat and c.observables()["step"] == 15
+    c.run(15)
+    for sp in (orbc.LIPID, orbc.PROTEIN):
+        sa, sc = a.get_state(sp), c.get_state(sp)
+        for k in ("x", "v", "n", "o"):
+            assert np.abs(sa[k] - sc[k]).max() < 1e-9 * max(1.0, np.abs(sa[k]).max()), (sp, k)
+    assert abs(a.observables()["zeta"] - c.observables()["zeta"]) < 1e-9
+
+
+def test_restart_with_cells_and_volume(tmp_path):
+    """The Voronoi centroids and every particle's cell come back, so a run with
+    the in-loop volume constraint resumes along the same trajectory."""
+    sphere = orbc.random_sphere(r=12.0, seed=2)
+
+    def make():
+        sim = orbc.Simulation(precision=orbc.FP64, seed=11, skin=0.3, kBT=0.22, volume_target=-1.0,
+                              volume_strength=0.5, freq_cleanup=0)
+        orbc.system_to_simulation(sphere, sim, kT=0.22, tilt=0.02, seed=3)
+        sim.voronoi_init(0, 4)
+        return sim
+
+    a = make()
+    a.run(21)
+    b = make()
+    b.run(11)                     # stops between two Voronoi updates
+    path = str(tmp_path / "v.rst")
+    b.save_restart(path)
+    c = orbc.Simulation()
+    c.load_restart(path)
+    c.run(10)
+    sa, sc = a.get_state(orbc.LIPID), c.get_state(orbc.LIPID)
+    for k in ("x", "v", "n"):
+        assert np.abs(sa[k] - sc[k]).max() < 1e-9 * max(1.0, np.abs(sa[k]).max()), k
+    ca, la, _ = a.voronoi_get()
+    cc, lc, _ = c.voronoi_get()
+    assert np.abs(ca - cc).max() < 1e-5 and np.array_equal(la, lc)
+
+
 def test_run_volume_needs_cells():
     sphere = orbc.random_sphere(r=8.0, seed=2)
     sim = orbc.Simulation(precision=orbc.FP64, volume_target=-1.0)
