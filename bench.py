@@ -315,10 +315,12 @@ def main():
                          kernel_launches=tm["step_launches"], rebuild_ms=tm["rebuild_ms"], rebuilds=rebuilds,
                          bytes_per_step=B, achieved_GBs=B / (kern_ms * 1e-3) / 1e9,
                          temperature=obs["temperature"], potential_per_particle=obs["potential"] / n,
-                         energy_per_particle=dict(start=e0 / n, end=e1 / n, drift=(e1 - e0) / n))
+                         energy_per_particle=dict(start=e0 / n, end=e1 / n, drift=(e1 - e0) / n),
+                         halo_blocks=obs["halo_blocks"], halo_overflow=obs["halo_overflow"])
         print(f"# {name}: {res[name]['value']:.4g} particle-steps/s, {res[name]['ms_per_step']:.3f} ms/step, "
               f"kernel {kern_ms:.3f} ms, rebuilds {rebuilds} ({tm['rebuild_ms']:.1f} ms), "
-              f"T={obs['temperature']:.4f}, E/N {e0 / n:.5f} -> {e1 / n:.5f}", file=sys.stderr, flush=True)
+              f"T={obs['temperature']:.4f}, E/N {e0 / n:.5f} -> {e1 / n:.5f}, "
+              f"halo blocks {obs['halo_blocks']} (overflow {obs['halo_overflow']})", file=sys.stderr, flush=True)
 
     # error again on the configuration reached by the mixed dynamics
     err_dyn = None
@@ -360,6 +362,7 @@ def main():
                         f"volume (strength {args.volume_strength})",
             "n_particles": n, "n_bonds": nb, "pairs_within_cutoff": e_pairs, "skin": args.skin,
             "precision": hname, "mesh": args.mesh,
+            "block_halos": {"blocks": head["halo_blocks"], "overflow": head["halo_overflow"]},
         },
         "roofline": {
             "bound": "hbm",

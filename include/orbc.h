@@ -96,6 +96,9 @@ typedef struct orbc_config {
     double volume_strength;     /* constrain_volume strength */
     int device;                 /* HIP device ordinal */
     int block_size;             /* force/step kernel block size: 64/128/256/512; 0 = tuned default */
+    int block_halo;             /* mixed Langevin step: 1 (default) stages each particle block's
+                                   neighbours in LDS and walks 16-bit local indices; 0 gathers every
+                                   neighbour from the 32-bit list.  Same forces either way. */
 } orbc_config;
 
 typedef struct orbc_observables {
@@ -116,6 +119,9 @@ typedef struct orbc_observables {
     int anchor_bits[3];
     long long n_lipid, n_protein;   /* species counts (lipids shrink with the stray-lipid cleanup) */
     long long lipids_deleted;       /* lipids removed by the cleanup so far */
+    long long halo_blocks;          /* particle blocks of the mixed step with an LDS halo at the last
+                                       rebuild (0 when block_halo is off or in FP64) */
+    long long halo_overflow;        /* blocks whose neighbours did not fit (32-bit gathers) */
 } orbc_observables;
 
 void orbc_config_default(orbc_config *cfg);

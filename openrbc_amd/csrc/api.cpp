@@ -506,6 +506,25 @@ void rebuild(orbc_ctx *c) {
         c->kmax = (int)mc + 16;
         c->nbr = c->mem.alloc<uint32_t>((size_t)c->kmax * c->stride);
     }
+    // 7b. block halos of the mixed step
+    c->halo_ok = false;
+    if (c->mixed() && c->cfg.block_halo) {
+        const long long nb = (n + BLOCK - 1) / BLOCK;
+        const long long words = halo_words(c->kmax) * c->stride;
+        if (nb > c->halo_nblk_cap) {
+            c->mem.release(c->halo_ext); c->mem.release(c->halo_cnt);
+            c->halo_nblk_cap = std::max(nb, (c->stride + BLOCK - 1) / BLOCK);
+            c->halo_ext = c->mem.alloc<uint32_t>((size_t)c->halo_nblk_cap * HALO_EXT);
+            c->halo_cnt = c->mem.alloc<uint32_t>((size_t)c->halo_nblk_cap);
+        }
+        if (words > c->nbr16_cap) {
+            c->mem.release(c->nbr16);
+            c->nbr16_cap = words;
+            c->nbr16 = c->mem.alloc<uint32_t>((size_t)words);
+        }
+        launch_halo(n, c->stride, c->nbr, c->nnbr, c->halo_ext, c->halo_cnt, c->nbr16, s);
+        c->halo_ok = true;
+    }
     // 8. bond lists
     if (c->nbonds > 0)
         launch_bondlist(n, c->stride, c->slot, c->inv, c->bstart, c->bpartner, c->btype_d, c->bnd, c->nbnd, s);
@@ -611,6 +630,7 @@ void orbc_config_default(orbc_config *cfg) {
     cfg->volume_strength = 0.05;   // src/openrbc.cpp:229
     cfg->device = 0;
     cfg->block_size = 0;
+    cfg->block_halo = 1;
 }
 
 static bool config_valid(const orbc_config &cfg_) {
@@ -954,6 +974,7 @@ static StepArgs step_args(orbc_ctx *c) {
     a.pq_out = c->pq[out];
     a.type = c->type; a.id = c->id; a.anc = c->anc; a.rb = c->rb;
     a.nbr = c->nbr; a.nnbr = c->nnbr;
+    if (c->halo_ok && c->mixed()) { a.halo_ext = c->halo_ext; a.halo_cnt = c->halo_cnt; a.nbr16 = c->nbr16; }
     a.bnd = c->nbonds ? c->bnd : nullptr;
     a.nbnd = c->nbonds ? c->nbnd : nullptr;
     a.has_bonds = c->nbonds > 0;
@@ -1247,6 +1268,14 @@ int orbc_get_observables(orbc_ctx *c, orbc_observables *obs) {
         obs->lipids_deleted = c->deleted;
         obs->step = c->step;
         obs->rebuilds = c->rebuilds;
+        if (c->halo_ok && c->n > 0) {
+            const long long nb = (c->n + BLOCK - 1) / BLOCK;
+            std::vector<uint32_t> hc(nb);
+            ORBC_HIP_CHECK(hipMemcpyAsync(hc.data(), c->halo_cnt, 4 * nb, hipMemcpyDeviceToHost, c->stream));
+            sync(c);
+            obs->halo_blocks = nb;
+            for (uint32_t v : hc) obs->halo_overflow += v == HALO_OVERFLOW;
+        }
         // directed list entries
         std::vector<uint32_t> cnt(c->n);
         ORBC_HIP_CHECK(hipMemcpyAsync(cnt.data(), c->nnbr, 4 * c->n, hipMemcpyDeviceToHost, c->stream));

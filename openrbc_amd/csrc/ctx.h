@@ -89,6 +89,10 @@ struct orbc_ctx {
     long long ncell_cap = 0;
     uint32_t *nbr = nullptr, *nnbr = nullptr;
     int kmax = 0;
+    // block halos of the mixed step (launch_halo), valid after a rebuild when halo_ok
+    uint32_t *halo_ext = nullptr, *halo_cnt = nullptr, *nbr16 = nullptr;
+    long long halo_nblk_cap = 0, nbr16_cap = 0;
+    bool halo_ok = false;
     uint32_t *bnd = nullptr;
     uint8_t *nbnd = nullptr;
     int kbmax = 0;

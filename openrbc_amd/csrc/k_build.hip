@@ -585,9 +585,120 @@ __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, co
         atomicMax(disp_max, __float_as_uint(d2));
 }
 
+// ---- block halos ------------------------------------------------------------
+// One workgroup per particle block b (particles i0 = b * BLOCK ...): the
+// distinct list entries outside the block go into an LDS hash set, and the
+// order in which they first arrive numbers them: ext[b * HALO_EXT + e] is
+// entry e.  Then each particle's entries are rewritten as local indices
+// (t < BLOCK: particle i0 + t; BLOCK + e: ext entry e), two 16-bit entries per
+// word, word m of particle i at nbr16[m * stride + i], rows padded to whole
+// groups of HALO_GROUP entries with the particle's own local index (distance
+// 0: no contribution).  Which ext slot an entry gets depends on the order of
+// the LDS atomics; the values staged at those slots, and so the forces, do
+// not.  The set stops taking entries once HALO_EXT distinct ones are in, so
+// at most HALO_EXT + BLOCK of its HALO_HASH slots fill and every probe
+// sequence ends at an empty or equal slot.
+constexpr int HALO_HASH = 4096;
+constexpr uint32_t HALO_EMPTY = 0xffffffffu;
+static_assert(HALO_EXT + BLOCK < HALO_HASH, "halo hash set too small");
+static_assert(HALO_CAP == BLOCK + HALO_EXT && HALO_CAP <= 65536, "16-bit halo indices");
+
+__device__ inline uint32_t halo_hash(uint32_t j) { return (j * 2654435761u) >> (32 - 12); }   // 12 = log2(HALO_HASH)
+
+__global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, const uint32_t *nbr,
+                                                const uint32_t *nnbr, uint32_t *ext, uint32_t *ext_cnt,
+                                                uint32_t *nbr16) {
+    __shared__ uint32_t s_key[HALO_HASH];
+    __shared__ uint16_t s_val[HALO_HASH];
+    __shared__ int s_cnt, s_over;
+    const int t = threadIdx.x;
+    const long long b = blockIdx.x, i0 = b * BLOCK, i = i0 + t;
+    for (int h = t; h < HALO_HASH; h += BLOCK) s_key[h] = HALO_EMPTY;
+    if (t == 0) { s_cnt = 0; s_over = 0; }
+    __syncthreads();
+    const int nn = i < n ? (int)nnbr[i] : 0;
+    const uint32_t self = (uint32_t)i;
+    uint32_t *ext_b = ext + b * HALO_EXT;
+    // entries in batches of HB (the loads of a batch in flight together);
+    // own-block entries and the padding (the particle itself) are skipped
+    constexpr int HB = 8;
+    for (int k0 = 0; k0 < nn; k0 += HB) {
+        uint32_t jb[HB];
+#pragma unroll
+        for (int u = 0; u < HB; ++u) jb[u] = (k0 + u < nn) ? nbr[(long long)(k0 + u) * stride + i] : self;
+        if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+#pragma unroll
+        for (int u = 0; u < HB; ++u) {
+            const uint32_t j = jb[u];
+            if ((unsigned long long)((long long)j - i0) < (unsigned long long)BLOCK) continue;
+            uint32_t h = halo_hash(j);
+            for (int probe = 0; probe < HALO_HASH; ++probe) {
+                const uint32_t prev = atomicCAS(&s_key[h], HALO_EMPTY, j);
+                if (prev == HALO_EMPTY) {
+                    const int e = atomicAdd(&s_cnt, 1);
+                    if (e < HALO_EXT) {
+                        s_val[h] = (uint16_t)e;
+                        ext_b[e] = j;
+                    } else {
+                        atomicExch(&s_over, 1);
+                    }
+                    break;
+                }
+                if (prev == j) break;
+                h = (h + 1) & (HALO_HASH - 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (s_over) {
+        if (t == 0) ext_cnt[b] = HALO_OVERFLOW;
+        return;
+    }
+    if (t == 0) ext_cnt[b] = (uint32_t)s_cnt;
+    if (i >= n) return;
+    // local indices, HB entries (HB / 2 words) per batch; the row's words
+    // [0, halo_row_words(nn)) are written, entries past nn being the particle
+    // itself, and nn <= kmax keeps them inside halo_words(kmax)
+    const int nw = halo_row_words(nn);
+    for (int m0 = 0; m0 < nw; m0 += HB / 2) {
+        uint32_t jb[HB];
+#pragma unroll
+        for (int u = 0; u < HB; ++u) jb[u] = (m0 * 2 + u < nn) ? nbr[(long long)(m0 * 2 + u) * stride + i] : self;
+#pragma unroll
+        for (int q = 0; q < HB / 2; ++q) {
+            if (m0 + q >= nw) break;
+            uint32_t word = 0;
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const uint32_t j = jb[2 * q + hh];
+                const long long d = (long long)j - i0;
+                uint32_t l = (uint32_t)t;
+                if (d >= 0 && d < BLOCK) {
+                    l = (uint32_t)d;
+                } else {
+                    // j is in the set (no overflow): the probe ends at its slot
+                    uint32_t h = halo_hash(j);
+                    for (int probe = 0; probe < HALO_HASH; ++probe) {
+                        if (s_key[h] == j) { l = (uint32_t)BLOCK + s_val[h]; break; }
+                        h = (h + 1) & (HALO_HASH - 1);
+                    }
+                }
+                word |= l << (16 * hh);
+            }
+            nbr16[(long long)(m0 + q) * stride + i] = word;
+        }
+    }
+}
+
 inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
 }  // namespace
+
+void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, uint32_t *ext,
+                 uint32_t *cnt, uint32_t *nbr16, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, ext, cnt, nbr16);
+}
 
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g, unsigned *disp_max,
                  hipStream_t s) {

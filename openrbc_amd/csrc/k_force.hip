@@ -290,6 +290,17 @@ constexpr int UNROLL64 = ORBC_FP64_UNROLL;
 #else
 #define ORBC_STEP_ATTR
 #endif
+// waves per SIMD of the block-halo step kernel: its LDS (HALO_CAP mirrors)
+// admits 5 blocks of 256 per CU at HALO_CAP = 896, and 96 registers hold it
+// without spills
+#ifndef ORBC_HALO_WAVES
+#define ORBC_HALO_WAVES 5
+#endif
+#if ORBC_HALO_WAVES > 0
+#define ORBC_HALO_ATTR __attribute__((amdgpu_waves_per_eu(ORBC_HALO_WAVES)))
+#else
+#define ORBC_HALO_ATTR
+#endif
 
 // ---- per-particle force accumulation ---------------------------------------
 // The list is walked in groups of UNROLL entries, loading the next group's
@@ -324,14 +335,15 @@ __device__ __forceinline__ void pair_mixed(const Anchor &ag, const AnchorI &ai, 
 #ifndef ORBC_PIPE
 #define ORBC_PIPE 1
 #endif
-template <bool ENERGY>
+template <bool ENERGY, int U = UNROLL, bool PIPE = (ORBC_PIPE != 0)>
 __device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long long stride, const PQ *pq,
                                             const uint32_t *nb, const Anchor &ag, const AnchorI &ai, float4 xi,
                                             float4 ni, int ti, const FFS<float> &t, Acc<double> &out) {
+    constexpr int UNROLL = U;
     uint32_t jn[UNROLL];
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) jn[u] = (kb + u < ke) ? nb[(long long)(kb + u) * stride] : (uint32_t)i;
-    if constexpr (ORBC_PIPE) {
+    if constexpr (PIPE) {
         float4 xa[UNROLL], na[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) { xa[u] = pq[jn[u]].x; na[u] = pq[jn[u]].n; }
@@ -388,6 +400,88 @@ __device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long lo
     }
 }
 
+// Mixed walk over a block halo in LDS (k_halo): 16-bit local indices, two
+// per word, in groups of HALO_GROUP entries (rows padded with the particle's
+// own index); the next group's words are loaded while this group is
+// evaluated.  The terms and their order are those of range_mixed over the
+// 32-bit list.
+#ifndef ORBC_HALO_STREAM
+#define ORBC_HALO_STREAM 1
+#endif
+template <bool ENERGY>
+__device__ __forceinline__ void range_halo(int nn, long long stride, const uint32_t *w16, const float4 *sx,
+                                           const float4 *sn, const Anchor &ag, const AnchorI &ai, float4 xi,
+                                           float4 ni, int ti, const FFS<float> &t, Acc<double> &out) {
+    if constexpr (ORBC_HALO_STREAM && ORBC_PAIR_F64 >= 2) {
+        // one word (two entries) per iteration, each entry's LDS reads issued
+        // one entry ahead of its use, words loaded two ahead: 16 registers of
+        // staged neighbours instead of a whole group's
+        const int nw = halo_row_words(nn);
+        if (nw == 0) return;
+        uint32_t w = w16[0];
+        uint32_t wn = (nw > 1) ? w16[stride] : 0u;
+        float4 xa = sx[w & 0xffffu], na = sn[w & 0xffffu];
+        for (int m = 0; m < nw; ++m) {
+            const uint32_t wc = w;
+            w = wn;
+            if (m + 2 < nw) wn = w16[(long long)(m + 2) * stride];
+            const uint32_t lb = wc >> 16;
+            const float4 xb = sx[lb], nb = sn[lb];
+            pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xa, na, t, out);
+            if (m + 1 < nw) { xa = sx[w & 0xffffu]; na = sn[w & 0xffffu]; }
+            pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xb, nb, t, out);
+        }
+        return;
+    }
+    constexpr int G = HALO_GROUP, W = G / 2;
+    const int nw = halo_row_words(nn);
+    uint32_t wn[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) wn[q] = (nw > 0) ? w16[(long long)q * stride] : 0u;
+    for (int m = 0; m < nw; m += W) {
+        uint32_t wc[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) wc[q] = wn[q];
+        if (m + W < nw) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) wn[q] = w16[(long long)(m + W + q) * stride];
+        }
+        float4 xj[G], nj[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const uint32_t l = (u & 1) ? (wc[u >> 1] >> 16) : (wc[u >> 1] & 0xffffu);
+            xj[u] = sx[l]; nj[u] = sn[l];
+        }
+        Acc<float> acc32;
+        Acc<PairAcc> &acc = *[&]() {
+            if constexpr (ORBC_PAIR_F64 >= 2) return &out; else return &acc32;
+        }();
+#pragma unroll
+        for (int u = 0; u < G; ++u) pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xj[u], nj[u], t, acc);
+        if constexpr (ORBC_PAIR_F64 < 2) add_to(out, acc32);
+    }
+}
+
+// compute_bonded from the FP32 mirrors (f only)
+template <bool ENERGY>
+__device__ __forceinline__ void bonds_mixed(long long i, long long stride, const PQ *pq, const uint32_t *bnd,
+                                            const uint8_t *nbnd, const Anchor &ag, const AnchorI &ai, float4 xi,
+                                            const FFS<float> &t, Acc<double> &out) {
+    const int nb = nbnd[i];
+    Acc<float> acc;
+    for (int k = 0; k < nb; ++k) {
+        const uint32_t e = bnd[(long long)k * stride + i];
+        const uint32_t j = e & BOND_INDEX_MASK;
+        const float4 xj = pq[j].x;
+        const AnchorI aj = unpack(__float_as_uint(xj.w), ag);
+        const float dx = anchor_add(ag, 0, sub_rn(xj.x, xi.x), aj.x - ai.x);
+        const float dy = anchor_add(ag, 1, sub_rn(xj.y, xi.y), aj.y - ai.y);
+        const float dz = anchor_add(ag, 2, sub_rn(xj.z, xi.z), aj.z - ai.z);
+        bond_ij<float, ENERGY>(t, (int)(e >> 30), dx, dy, dz, acc);
+    }
+    add_to(out, acc);
+}
+
 template <bool ENERGY>
 __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, const PQ *pq,
                                                  const uint32_t *nbr, const uint32_t *nnbr,
@@ -399,21 +493,7 @@ __device__ __forceinline__ void accumulate_mixed(long long i, long long stride, 
     if (pair) {
         range_mixed<ENERGY>(i, 0, (int)nnbr[i], stride, pq, nbr + i, ag, ai, xi, ni, ti, t, out);
     }
-    if (bond && nbnd) {
-        const int nb = nbnd[i];
-        Acc<float> acc;
-        for (int k = 0; k < nb; ++k) {
-            const uint32_t e = bnd[(long long)k * stride + i];
-            const uint32_t j = e & BOND_INDEX_MASK;
-            const float4 xj = pq[j].x;
-            const AnchorI aj = unpack(__float_as_uint(xj.w), ag);
-            const float dx = anchor_add(ag, 0, sub_rn(xj.x, xi.x), aj.x - ai.x);
-            const float dy = anchor_add(ag, 1, sub_rn(xj.y, xi.y), aj.y - ai.y);
-            const float dz = anchor_add(ag, 2, sub_rn(xj.z, xi.z), aj.z - ai.z);
-            bond_ij<float, ENERGY>(t, (int)(e >> 30), dx, dy, dz, acc);
-        }
-        add_to(out, acc);
-    }
+    if (bond && nbnd) bonds_mixed<ENERGY>(i, stride, pq, bnd, nbnd, ag, ai, xi, t, out);
 }
 
 // FP64: one gathered neighbour against owner i (minimum image on periodic axes)
@@ -608,8 +688,12 @@ __device__ inline void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, u
 // src/openrbc.cpp:219-233 (compute_pairwise_fused, compute_bonded,
 // integrate(verlet_langevin)) in one pass; the Langevin update restates
 // src/integrate_langevin.h:275-297 in FP64 with counter-based noise.
-template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs a) {
+// HALO (mixed only): the block's neighbours are staged in LDS from its halo
+// (k_halo) and the list is walked as 16-bit local indices; a block marked
+// HALO_OVERFLOW gathers from the 32-bit list, one neighbour at a time.
+template <bool MIXED, bool HALO>
+__device__ __forceinline__ void step_langevin(const StepArgs &a) {
+    static_assert(MIXED || !HALO, "block halos hold FP32 mirrors");
     const int k3 = (int)(a.step % 3ull);
     // skin flag of this step's input positions; no block of this launch writes it
     if (*a.stop || a.dslot[(k3 + 2) % 3] != 0u) {
@@ -622,8 +706,26 @@ __global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs
     }
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
     load_ffs(&tab);
-    const long long i = block_index(a.swizzle) * BLOCK + threadIdx.x;
+    const long long blk = block_index(a.swizzle);
+    const long long i = blk * BLOCK + threadIdx.x;
     const bool act = i < a.n;
+    // block halo: own mirrors at [0, BLOCK), outside neighbours after them
+    __shared__ float4 s_hx[HALO ? HALO_CAP : 1], s_hn[HALO ? HALO_CAP : 1];
+    uint32_t hcnt = HALO_OVERFLOW;
+    PQ qi{};
+    if constexpr (HALO) {
+        hcnt = a.halo_cnt[blk];
+        if (act) qi = a.pq_in[i];
+        if (hcnt != HALO_OVERFLOW) {
+            s_hx[threadIdx.x] = qi.x; s_hn[threadIdx.x] = qi.n;
+            const uint32_t *ext = a.halo_ext + blk * HALO_EXT;
+            for (uint32_t q = threadIdx.x; q < hcnt; q += BLOCK) {
+                const PQ p = a.pq_in[ext[q]];
+                s_hx[BLOCK + q] = p.x; s_hn[BLOCK + q] = p.n;
+            }
+            __syncthreads();
+        }
+    }
     float d2 = 0.f;
     bool bad = false;
     if (act) {
@@ -631,7 +733,19 @@ __global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs
         int ti;
         uint32_t wi;
         Acc<double> acc;
-        if constexpr (MIXED) {
+        if constexpr (HALO) {
+            const float4 xi4 = qi.x, ni4 = qi.n;
+            ti = __float_as_int(ni4.w);
+            wi = __float_as_uint(xi4.w);
+            const AnchorI ai = unpack(wi, a.anc_grid);
+            const int nn = (int)a.nnbr[i];
+            if (hcnt != HALO_OVERFLOW)
+                range_halo<false>(nn, a.stride, a.nbr16 + i, s_hx, s_hn, a.anc_grid, ai, xi4, ni4, ti, tab, acc);
+            else
+                range_mixed<false, 1, false>(i, 0, nn, a.stride, a.pq_in, a.nbr + i, a.anc_grid, ai, xi4, ni4, ti,
+                                             tab, acc);
+            if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, a.anc_grid, ai, xi4, tab, acc);
+        } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
             const float4 xi4 = qi.x, ni4 = qi.n;
             accumulate_mixed<false>(i, a.stride, a.pq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, xi4,
@@ -702,6 +816,14 @@ __global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs
     const unsigned long long over = __ballot(d2 > a.half_skin2);
     if (over && __lane_id() == (int)__ffsll((long long)over) - 1) atomicOr(a.dslot + k3, 1u);
     if (bad) atomicOr(a.bad, 1u);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs a) {
+    step_langevin<MIXED, false>(a);
+}
+__global__ __launch_bounds__(BLOCK) ORBC_HALO_ATTR void k_step_langevin_halo(StepArgs a) {
+    step_langevin<true, true>(a);
 }
 
 // ---- fused forces + post_toque_final_update (NH second half) --------------
@@ -786,7 +908,8 @@ void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool 
 void launch_step_langevin(const StepArgs &a, bool mixed, hipStream_t s) {
     if (a.n <= 0) return;
     const dim3 g(nblk(a.n)), b(BLOCK);
-    if (mixed) hipLaunchKernelGGL(k_step_langevin<true>, g, b, 0, s, a);
+    if (mixed && a.halo_cnt) hipLaunchKernelGGL(k_step_langevin_halo, g, b, 0, s, a);
+    else if (mixed) hipLaunchKernelGGL(k_step_langevin<true>, g, b, 0, s, a);
     else hipLaunchKernelGGL(k_step_langevin<false>, g, b, 0, s, a);
 }
 

@@ -125,6 +125,27 @@ struct PQ {
     float4 x, n;
 };
 
+// Block halos of the mixed step (k_halo, k_step_langevin): particle block b
+// (the BLOCK particles from b * BLOCK in store order) stages its own mirrors
+// and those of at most HALO_EXT distinct outside neighbours in LDS, and walks
+// its list as 16-bit indices into that table.  A block with more outside
+// neighbours is marked HALO_OVERFLOW and gathers from the 32-bit list.
+#ifndef ORBC_HALO_CAP
+#define ORBC_HALO_CAP 896
+#endif
+constexpr int HALO_CAP = ORBC_HALO_CAP;        // local indices: BLOCK own + HALO_EXT outside
+constexpr int HALO_EXT = HALO_CAP - BLOCK;
+constexpr uint32_t HALO_OVERFLOW = 0xffffffffu;
+// entries per group of the LDS walk (rows are padded to whole groups)
+#ifndef ORBC_HALO_GROUP
+#define ORBC_HALO_GROUP 4
+#endif
+constexpr int HALO_GROUP = ORBC_HALO_GROUP;
+static_assert(HALO_GROUP == 4 || HALO_GROUP == 8, "halo group of 4 or 8 entries");
+// 32-bit words of 16-bit entries in a row of nn entries (whole groups)
+__host__ __device__ inline int halo_row_words(int nn) { return (HALO_GROUP / 2) * ((nn + HALO_GROUP - 1) / HALO_GROUP); }
+inline long long halo_words(int kmax) { return halo_row_words(kmax); }
+
 // Everything a step kernel reads or writes, by value in kernel arguments.
 struct StepArgs {
     long long n;
@@ -140,6 +161,8 @@ struct StepArgs {
     const uint32_t *anc;       // packed anchors (FP64 path)
     const float *rb;           // offsets at the last build (skin check)
     const uint32_t *nbr, *nnbr;
+    // block halos (mixed path; null: every block gathers from nbr)
+    const uint32_t *halo_ext, *halo_cnt, *nbr16;
     const uint32_t *bnd;
     const uint8_t *nbnd;
     int has_bonds;
@@ -250,6 +273,10 @@ void launch_disp(long long n, const double *x, const uint32_t *anc, const float 
 void launch_bondlist(long long n, long long stride, const uint32_t *slot, const uint32_t *inv,
                      const uint32_t *bstart, const uint32_t *bpartner, const uint8_t *btype,
                      uint32_t *bnd, uint8_t *nbnd, hipStream_t s);
+// block halos from the 32-bit list: ext (HALO_EXT per block), cnt (per block),
+// nbr16 (halo_words(kmax) words per particle, column stride `stride`)
+void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, uint32_t *ext,
+                 uint32_t *cnt, uint32_t *nbr16, hipStream_t s);
 
 // k_force.hip
 void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool bond, hipStream_t s);

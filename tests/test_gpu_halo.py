@@ -1,0 +1,105 @@
+"""Block halos of the mixed Langevin step.
+
+With ``block_halo`` on, each block of 256 particles stages its own FP32
+mirrors and those of its outside neighbours in LDS and walks its list as
+16-bit local indices; a block whose outside neighbours do not fit gathers from
+the 32-bit list.  The pair terms and their order are the same either way, so a
+run with halos must be bit-identical to a run without them, through neighbour
+rebuilds, with and without blocks that overflow.
+"""
+import numpy as np
+import pytest
+
+import openrbc_amd as orbc
+
+pytestmark = pytest.mark.gpu
+
+
+def _rbc(level=2, seed=1):
+    """Icosphere cell through the initialiser: lipids, the four protein types
+    and the spectrin network with its bonds."""
+    v, e, f = orbc.icosphere(level)
+    s = orbc.init_rbc(v, e, f, seed=seed)
+    rng = np.random.default_rng(seed)
+    x = np.vstack([s["lipid_x"], s["protein_x"]])
+    n = np.vstack([s["lipid_n"], s["protein_n"]])
+    t = np.concatenate([np.zeros(len(s["lipid_x"]), np.int32), s["protein_type"]])
+    ids = np.concatenate([s["lipid_id"], s["protein_id"]])
+    idx = {int(k): i for i, k in enumerate(ids)}
+    bi = np.array([idx[int(k)] for k in s["bond_i"]]); bj = np.array([idx[int(k)] for k in s["bond_j"]])
+    return dict(x=x, n=n, type=t, id=ids, v=rng.normal(0, 0.3, x.shape), o=np.zeros_like(x),
+                bonds=(s["bond_type"], bi, bj))
+
+
+def _with_blob(s, m=16, spacing=1.0, offset=(400.0, 0.0, 0.0), seed=3):
+    """Append an m^3 cubic lattice of lipids (about 140 list entries each, so
+    a block's outside neighbours overflow the halo) far from the patch."""
+    rng = np.random.default_rng(seed)
+    g = np.arange(m) * spacing
+    p = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + np.array(offset)
+    p += rng.normal(0, 0.02, p.shape)
+    nb = len(p)
+    nrm = rng.normal(0, 1, (nb, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    out = dict(s)
+    out["x"] = np.vstack([s["x"], p])
+    out["n"] = np.vstack([s["n"], nrm])
+    out["v"] = np.vstack([s["v"], rng.normal(0, 0.1, (nb, 3))])
+    out["o"] = np.vstack([s["o"], rng.normal(0, 0.05, (nb, 3))])
+    out["type"] = np.concatenate([s["type"], np.zeros(nb, np.int32)])
+    out["id"] = np.concatenate([s["id"], np.arange(nb, dtype=np.uint32) + s["id"].max() + 1])
+    return out
+
+
+def _run(s, halo, nsteps, minimize=0, **kw):
+    sim = orbc.Simulation(precision=orbc.MIXED, block_halo=halo, **kw)
+    t = s["type"]
+    lip = np.where(t == 0)[0]
+    prot = np.where(t != 0)[0]
+    sim.set_state(orbc.LIPID, s["x"][lip], s["n"][lip], s["id"][lip], v=s["v"][lip], o=s["o"][lip])
+    if len(prot):
+        sim.set_state(orbc.PROTEIN, s["x"][prot], s["n"][prot], s["id"][prot], v=s["v"][prot], o=s["o"][prot],
+                      type=t[prot])
+    bt, bi, bj = s["bonds"]
+    if len(bt):
+        sim.set_bonds(bt, s["id"][bi], s["id"][bj])
+    if minimize:
+        sim.minimize(minimize)
+    sim.run(nsteps)
+    st = {sp: sim.get_state(sp) for sp in (orbc.LIPID, orbc.PROTEIN) if (len(lip) if sp == orbc.LIPID else len(prot))}
+    obs = sim.observables()
+    sim.close()
+    return st, obs
+
+
+def _same(a, b):
+    for sp in a:
+        for k in a[sp]:
+            assert np.array_equal(a[sp][k], b[sp][k]), (sp, k)
+
+
+@pytest.fixture(scope="module")
+def rbc():
+    return _rbc(2)
+
+
+@pytest.mark.parametrize("skin", [0.05, 0.4])
+def test_block_halo_run_bit_identical(rbc, skin):
+    s = rbc
+    print(f"halo run: icosphere-2 cell, n={len(s['type'])}, skin {skin}", flush=True)
+    on, obs_on = _run(s, 1, 40, minimize=30, seed=5, skin=skin)
+    off, obs_off = _run(s, 0, 40, minimize=30, seed=5, skin=skin)
+    _same(on, off)
+    assert obs_on["rebuilds"] == obs_off["rebuilds"] and obs_on["rebuilds"] > 1
+    assert obs_on["halo_blocks"] == (len(s["type"]) + 255) // 256
+    assert obs_on["halo_overflow"] < obs_on["halo_blocks"]
+    assert obs_off["halo_blocks"] == 0
+
+
+def test_block_halo_overflow_blocks_gather_globally(rbc):
+    s = _with_blob(rbc)
+    print(f"halo overflow run: n={len(s['type'])}", flush=True)
+    on, obs_on = _run(s, 1, 6, minimize=20, seed=4, skin=0.2)
+    off, _ = _run(s, 0, 6, minimize=20, seed=4, skin=0.2)
+    _same(on, off)
+    assert 0 < obs_on["halo_overflow"] < obs_on["halo_blocks"]
