@@ -522,7 +522,7 @@ void rebuild(orbc_ctx *c) {
             c->nbr16_cap = words;
             c->nbr16 = c->mem.alloc<uint32_t>((size_t)words);
         }
-        launch_halo(n, c->stride, c->nbr, c->nnbr, c->halo_ext, c->halo_cnt, c->nbr16, s);
+        launch_halo(n, c->stride, c->nbr, c->nnbr, c->type, c->halo_ext, c->halo_cnt, c->nbr16, s);
         c->halo_ok = true;
     }
     // 8. bond lists

@@ -589,11 +589,12 @@ __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, co
 // One workgroup per particle block b (particles i0 = b * BLOCK ...): the
 // distinct list entries outside the block go into an LDS hash set, and the
 // order in which they first arrive numbers them: ext[b * HALO_EXT + e] is
-// entry e.  Then each particle's entries are rewritten as local indices
-// (t < BLOCK: particle i0 + t; BLOCK + e: ext entry e), two 16-bit entries per
-// word, word m of particle i at nbr16[m * stride + i], rows padded to whole
-// groups of HALO_GROUP entries with the particle's own local index (distance
-// 0: no contribution).  Which ext slot an entry gets depends on the order of
+// entry e.  Then each particle's entries are rewritten as 16-bit entries: the
+// local index (t < BLOCK: particle i0 + t; BLOCK + e: ext entry e) in bits
+// 0-9 and the neighbour's type in bits 10-12, two per word, word m of
+// particle i at nbr16[m * stride + i], rows padded to whole groups of
+// HALO_GROUP entries with the particle's own local index and type 0
+// (distance 0: no contribution).  Which ext slot an entry gets depends on the order of
 // the LDS atomics; the values staged at those slots, and so the forces, do
 // not.  The set stops taking entries once HALO_EXT distinct ones are in, so
 // at most HALO_EXT + BLOCK of its HALO_HASH slots fill and every probe
@@ -601,13 +602,14 @@ __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, co
 constexpr int HALO_HASH = 4096;
 constexpr uint32_t HALO_EMPTY = 0xffffffffu;
 static_assert(HALO_EXT + BLOCK < HALO_HASH, "halo hash set too small");
-static_assert(HALO_CAP == BLOCK + HALO_EXT && HALO_CAP <= 65536, "16-bit halo indices");
+static_assert(HALO_CAP == BLOCK + HALO_EXT && HALO_CAP <= 1024, "10-bit halo indices");
+static_assert(NT <= 8, "3-bit types in the halo entries");
 
 __device__ inline uint32_t halo_hash(uint32_t j) { return (j * 2654435761u) >> (32 - 12); }   // 12 = log2(HALO_HASH)
 
 __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, const uint32_t *nbr,
-                                                const uint32_t *nnbr, uint32_t *ext, uint32_t *ext_cnt,
-                                                uint32_t *nbr16) {
+                                                const uint32_t *nnbr, const uint8_t *type, uint32_t *ext,
+                                                uint32_t *ext_cnt, uint32_t *nbr16) {
     __shared__ uint32_t s_key[HALO_HASH];
     __shared__ uint16_t s_val[HALO_HASH];
     __shared__ int s_cnt, s_over;
@@ -661,9 +663,11 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
     // itself, and nn <= kmax keeps them inside halo_words(kmax)
     const int nw = halo_row_words(nn);
     for (int m0 = 0; m0 < nw; m0 += HB / 2) {
-        uint32_t jb[HB];
+        uint32_t jb[HB], tb[HB];
 #pragma unroll
         for (int u = 0; u < HB; ++u) jb[u] = (m0 * 2 + u < nn) ? nbr[(long long)(m0 * 2 + u) * stride + i] : self;
+#pragma unroll
+        for (int u = 0; u < HB; ++u) tb[u] = (m0 * 2 + u < nn) ? (uint32_t)type[jb[u]] : 0u;
 #pragma unroll
         for (int q = 0; q < HB / 2; ++q) {
             if (m0 + q >= nw) break;
@@ -683,7 +687,7 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
                         h = (h + 1) & (HALO_HASH - 1);
                     }
                 }
-                word |= l << (16 * hh);
+                word |= (l | (tb[2 * q + hh] << 10)) << (16 * hh);
             }
             nbr16[(long long)(m0 + q) * stride + i] = word;
         }
@@ -694,10 +698,10 @@ inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); 
 
 }  // namespace
 
-void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, uint32_t *ext,
-                 uint32_t *cnt, uint32_t *nbr16, hipStream_t s) {
+void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
+                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, ext, cnt, nbr16);
+    hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, type, ext, cnt, nbr16);
 }
 
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g, unsigned *disp_max,

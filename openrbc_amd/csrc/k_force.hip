@@ -110,43 +110,54 @@ __device__ inline void radius(double r2, double &r, double &rinv) {
 #ifndef ORBC_PAIR_F64
 #define ORBC_PAIR_F64 2
 #endif
+// mode 2: the term in FP64 from the exact separation X, Y, Z (cut, rep, att
+// the FP64 constants, alpha the FP32 one, r2 the FP32 squared distance of the
+// selection test)
+template <bool ENERGY, typename AR>
+__device__ __forceinline__ void aniso_fp64(double cut, double rep, double att, float alpha_f, float r2, double X,
+                                           double Y, double Z, float nix, float niy, float niz, float njx, float njy,
+                                           float njz, Acc<AR> &acc) {
+    const double d2 = __fma_rn(Z, Z, __fma_rn(Y, Y, X * X));
+    double y = (double)rsq(r2);                               // 1/r to FP32 accuracy
+    y = y * __fma_rn(-0.5 * d2, y * y, 1.5);                  // one Newton step: FP64 accuracy
+    const double rd = d2 * y;
+    const double ex = X * y, ey = Y * y, ez = Z * y;
+    const double ax = nix, ay = niy, az = niz, bx = njx, by = njy, bz = njz;
+    const double ninj = __fma_rn(az, bz, __fma_rn(ay, by, ax * bx));
+    const double nie = __fma_rn(az, ez, __fma_rn(ay, ey, ax * ex));
+    const double nje = __fma_rn(bz, ez, __fma_rn(by, ey, bx * ex));
+    const double a = __fma_rn(-nie, nje, ninj);
+    const double pix = __fma_rn(-nie, ex, ax), piy = __fma_rn(-nie, ey, ay), piz = __fma_rn(-nie, ez, az);
+    const double pjx = __fma_rn(-nje, ex, bx), pjy = __fma_rn(-nje, ey, by), pjz = __fma_rn(-nje, ez, bz);
+    const double s = cut - rd;
+    const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+    const double alpha = alpha_f;
+    const double A = __fma_rn(alpha, a - 1.0, 1.0);
+    const double aua = alpha * (att * s4);
+    const double auar = aua * y;
+    const double fra = s3 * __fma_rn(4.0 * A, att, 8.0 * rep * s4);
+    const double fx = __fma_rn(fra, ex, auar * __fma_rn(nje, pix, nie * pjx));
+    const double fy = __fma_rn(fra, ey, auar * __fma_rn(nje, piy, nie * pjy));
+    const double fz = __fma_rn(fra, ez, auar * __fma_rn(nje, piz, nie * pjz));
+    acc.tx -= (AR)(aua * pjx);
+    acc.ty -= (AR)(aua * pjy);
+    acc.tz -= (AR)(aua * pjz);
+    acc.fx += (AR)fx; acc.fy += (AR)fy; acc.fz += (AR)fz;
+    if (ENERGY) {
+        acc.e += (AR)(0.5 * s4 * __fma_rn(A, att, rep * s4));
+        acc.w += (AR)(0.5 * __fma_rn(Z, fz, __fma_rn(Y, fy, X * fx)));
+    }
+}
+
 template <typename R, bool ENERGY, typename AR>
 __device__ __forceinline__ void aniso_term(const FFS<R> &t, int k, R r2, R dx, R dy, R dz, double X, double Y,
                                            double Z, R nix, R niy, R niz, R njx, R njy, R njz, Acc<AR> &acc) {
     if constexpr (std::is_same<R, float>::value && ORBC_PAIR_F64 >= 2) {
-        const double d2 = __fma_rn(Z, Z, __fma_rn(Y, Y, X * X));
-        double y = (double)rsq(r2);                               // 1/r to FP32 accuracy
-        y = y * __fma_rn(-0.5 * d2, y * y, 1.5);                  // one Newton step: FP64 accuracy
-        const double rd = d2 * y;
-        const double ex = X * y, ey = Y * y, ez = Z * y;
-        const double ax = nix, ay = niy, az = niz, bx = njx, by = njy, bz = njz;
-        const double ninj = __fma_rn(az, bz, __fma_rn(ay, by, ax * bx));
-        const double nie = __fma_rn(az, ez, __fma_rn(ay, ey, ax * ex));
-        const double nje = __fma_rn(bz, ez, __fma_rn(by, ey, bx * ex));
-        const double a = __fma_rn(-nie, nje, ninj);
-        const double pix = __fma_rn(-nie, ex, ax), piy = __fma_rn(-nie, ey, ay), piz = __fma_rn(-nie, ez, az);
-        const double pjx = __fma_rn(-nje, ex, bx), pjy = __fma_rn(-nje, ey, by), pjz = __fma_rn(-nje, ez, bz);
-        const double s = t.cut_d[k] - rd;
-        const double s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
-        const double alpha = t.alpha[k];
-        const double A = __fma_rn(alpha, a - 1.0, 1.0);
-        const double rep = t.rep_d[k], att = t.att_d[k];
-        const double aua = alpha * (att * s4);
-        const double auar = aua * y;
-        const double fra = s3 * __fma_rn(4.0 * A, att, 8.0 * rep * s4);
-        const double fx = __fma_rn(fra, ex, auar * __fma_rn(nje, pix, nie * pjx));
-        const double fy = __fma_rn(fra, ey, auar * __fma_rn(nje, piy, nie * pjy));
-        const double fz = __fma_rn(fra, ez, auar * __fma_rn(nje, piz, nie * pjz));
-        acc.tx -= (AR)(aua * pjx);
-        acc.ty -= (AR)(aua * pjy);
-        acc.tz -= (AR)(aua * pjz);
-        acc.fx += (AR)fx; acc.fy += (AR)fy; acc.fz += (AR)fz;
-        if (ENERGY) {
-            acc.e += (AR)(0.5 * s4 * __fma_rn(A, att, rep * s4));
-            acc.w += (AR)(0.5 * __fma_rn(Z, fz, __fma_rn(Y, fy, X * fx)));
-        }
+        aniso_fp64<ENERGY>(t.cut_d[k], t.rep_d[k], t.att_d[k], t.alpha[k], r2, X, Y, Z, nix, niy, niz, njx, njy,
+                           njz, acc);
         return;
-    } else {
+    }
+    {
         R r, rinv;
         radius(r2, r, rinv);
         const R alpha = t.alpha[k];
@@ -201,6 +212,36 @@ __device__ __forceinline__ void aniso_term(const FFS<R> &t, int k, R r2, R dx, R
 // the pair interacts (same selection test as the oracle, bit for bit).
 // X, Y, Z: the separation in FP64 (the mixed path's exact mirror separation;
 // equal to dx, dy, dz in the FP64 path).
+// Repulsive protein-protein polynomial, src/pairwise_kernel_fused.h:79-97, and
+// WCA Lennard-Jones, :63-77, for the packed-record path (pair_rec); the same
+// arithmetic as the branches of pair_ij, the forces rounded before they are
+// summed.
+template <typename R, bool ENERGY, typename AR>
+__device__ __forceinline__ void pp_term(R cutp, R rep, R r2, R dx, R dy, R dz, Acc<AR> &acc) {
+    R r, rinv;
+    radius(r2, r, rinv);
+    const R ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
+    const R s = cutp - r;
+    const R s3 = s * s * s, s4 = (s * s) * (s * s);
+    const R fr = R(8) * rep * s3 * s4;
+    const R fx = fr * ex, fy = fr * ey, fz = fr * ez;
+    if (ENERGY) acc.e += (AR)(R(0.5) * rep * s4 * s4);
+    // rounded products, then the sums (no fused multiply-add across them)
+    acc.fx = add_rn(acc.fx, (AR)fx); acc.fy = add_rn(acc.fy, (AR)fy); acc.fz = add_rn(acc.fz, (AR)fz);
+    if (ENERGY) acc.w += (AR)(R(0.5) * (dx * fx + dy * fy + dz * fz));
+}
+
+template <typename R, bool ENERGY, typename AR>
+__device__ __forceinline__ void lj_term(R lj1, R lj2, R r2, R dx, R dy, R dz, Acc<AR> &acc) {
+    const R r2inv = R(1) / r2;
+    const R r6inv = r2inv * r2inv * r2inv;
+    const R fpair = r6inv * (lj1 * r6inv - lj2) * r2inv;
+    const R fx = dx * fpair, fy = dy * fpair, fz = dz * fpair;
+    if (ENERGY) acc.e += (AR)(R(0.5) * r6inv * (lj1 * (R(1) / R(12)) * r6inv - lj2 * (R(1) / R(6))));
+    acc.fx = add_rn(acc.fx, (AR)fx); acc.fy = add_rn(acc.fy, (AR)fy); acc.fz = add_rn(acc.fz, (AR)fz);
+    if (ENERGY) acc.w += (AR)(R(0.5) * (dx * fx + dy * fy + dz * fz));
+}
+
 template <typename R, bool ENERGY, typename AR>
 __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R dy, R dz, double X, double Y,
                                         double Z, R r2, R nix, R niy, R niz, R njx, R njy, R njz, Acc<AR> &acc) {
@@ -212,7 +253,7 @@ __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R
             aniso_term<R, ENERGY>(t, k, r2, dx, dy, dz, X, Y, Z, nix, niy, niz, njx, njy, njz, acc);
             return true;
         }
-        // src/pairwise_kernel_fused.h:79-97
+        // src/pairwise_kernel_fused.h:79-97 (pp_term)
         R r, rinv;
         radius(r2, r, rinv);
         const R ex = dx * rinv, ey = dy * rinv, ez = dz * rinv;
@@ -224,7 +265,7 @@ __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R
         if (ENERGY) acc.e += (AR)(R(0.5) * rep * s4 * s4);
     } else {
         if (!(r2 < t.cut2lj[k])) return false;
-        // src/pairwise_kernel_fused.h:63-77 (WCA, unshifted)
+        // src/pairwise_kernel_fused.h:63-77 (lj_term)
         const R r2inv = R(1) / r2;
         const R r6inv = r2inv * r2inv * r2inv;
         const R fpair = r6inv * (t.lj1[k] * r6inv - t.lj2[k]) * r2inv;
@@ -236,8 +277,55 @@ __device__ __forceinline__ bool pair_ij(const FFS<R> &t, int ti, int tj, R dx, R
     return true;
 }
 
-template <typename R, bool ENERGY>
-__device__ __forceinline__ void bond_ij(const FFS<R> &t, int bt, R dx, R dy, R dz, Acc<R> &acc) {
+// Pair-type record of the block-halo step: everything one type pair needs in
+// three 16-byte LDS words read together (FP32 values are the FP64 ones
+// rounded, as in the FP32 tables, so the terms are those of pair_ij).
+struct PairRec {
+    float4 sel;        // cut2 of the polynomial, cut2 of the WCA, alpha, anisotropic flag (int bits)
+    double2 poly;      // cut, rep (FP64)
+    double att;        // FP64
+    float lj1, lj2;
+};
+static_assert(sizeof(PairRec) == 48, "three 16-byte LDS words");
+
+__device__ inline void load_pair_recs(PairRec *s) {
+    for (int k = threadIdx.x; k < 36; k += blockDim.x) {
+        PairRec r;
+        r.sel = make_float4(c_fff.cut2_poly[k], c_fff.cut2_lj[k], c_fff.alpha[k], __int_as_float(c_fff.aniso[k]));
+        r.poly = make_double2(c_ffd.cut_poly[k], c_ffd.rep[k]);
+        r.att = c_ffd.att[k];
+        r.lj1 = c_fff.lj1[k]; r.lj2 = c_fff.lj2[k];
+        s[k] = r;
+    }
+}
+
+// pair_mixed (mode 2) with the type pair's constants from a PairRec
+__device__ __forceinline__ void pair_rec(const Anchor &ag, const AnchorI &ai, float4 xi, float4 ni, float4 xj,
+                                         float4 nj, const PairRec &P, Acc<double> &acc) {
+    const AnchorI aj = anchor_unpack(__float_as_uint(xj.w), ag);
+    const float dx = anchor_add(ag, 0, sub_rn(xi.x, xj.x), ai.x - aj.x);
+    const float dy = anchor_add(ag, 1, sub_rn(xi.y, xj.y), ai.y - aj.y);
+    const float dz = anchor_add(ag, 2, sub_rn(xi.z, xj.z), ai.z - aj.z);
+    const float r2 = r2_rn(dx, dy, dz);
+    if (!(r2 > 1e-5f)) return;
+    if (r2 < P.sel.x) {
+        if (__float_as_int(P.sel.w)) {
+            const double X = anchor_add_exact(ag, 0, xi.x, xj.x, ai.x - aj.x);
+            const double Y = anchor_add_exact(ag, 1, xi.y, xj.y, ai.y - aj.y);
+            const double Z = anchor_add_exact(ag, 2, xi.z, xj.z, ai.z - aj.z);
+            aniso_fp64<false>(P.poly.x, P.poly.y, P.att, P.sel.z, r2, X, Y, Z, ni.x, ni.y, ni.z, nj.x, nj.y, nj.z,
+                              acc);
+        } else {
+            pp_term<float, false>((float)P.poly.x, (float)P.poly.y, r2, dx, dy, dz, acc);
+        }
+        return;
+    }
+    if (!(r2 < P.sel.y)) return;
+    lj_term<float, false>(P.lj1, P.lj2, r2, dx, dy, dz, acc);
+}
+
+template <typename R, bool ENERGY, typename TAB>
+__device__ __forceinline__ void bond_ij(const TAB &t, int bt, R dx, R dy, R dz, Acc<R> &acc) {
     const R r2 = dx * dx + dy * dy + dz * dz;
     R r, rinv;
     radius(r2, r, rinv);
@@ -281,7 +369,7 @@ __device__ inline void add_to(Acc<double> &o, const Acc<R> &a) {
 #define ORBC_FP64_UNROLL 4
 #endif
 #ifndef ORBC_STEP_WAVES
-#define ORBC_STEP_WAVES 0
+#define ORBC_STEP_WAVES 4
 #endif
 constexpr int UNROLL = ORBC_STEP_UNROLL;
 constexpr int UNROLL64 = ORBC_FP64_UNROLL;
@@ -400,73 +488,39 @@ __device__ __forceinline__ void range_mixed(long long i, int kb, int ke, long lo
     }
 }
 
-// Mixed walk over a block halo in LDS (k_halo): 16-bit local indices, two
-// per word, in groups of HALO_GROUP entries (rows padded with the particle's
-// own index); the next group's words are loaded while this group is
-// evaluated.  The terms and their order are those of range_mixed over the
-// 32-bit list.
-#ifndef ORBC_HALO_STREAM
-#define ORBC_HALO_STREAM 1
-#endif
-template <bool ENERGY>
+// Mixed walk over a block halo in LDS (k_halo): 16-bit entries, two per
+// word (local index in bits 0-9, the neighbour's type in bits 10-12), rows
+// padded to whole groups of HALO_GROUP entries with the particle itself.
+// Words are loaded two ahead; each entry's mirrors are read from LDS one
+// entry ahead of its use, and its pair-type record as soon as its word is in.
+// The terms and their order are those of range_mixed over the 32-bit list.
 __device__ __forceinline__ void range_halo(int nn, long long stride, const uint32_t *w16, const float4 *sx,
-                                           const float4 *sn, const Anchor &ag, const AnchorI &ai, float4 xi,
-                                           float4 ni, int ti, const FFS<float> &t, Acc<double> &out) {
-    if constexpr (ORBC_HALO_STREAM && ORBC_PAIR_F64 >= 2) {
-        // one word (two entries) per iteration, each entry's LDS reads issued
-        // one entry ahead of its use, words loaded two ahead: 16 registers of
-        // staged neighbours instead of a whole group's
-        const int nw = halo_row_words(nn);
-        if (nw == 0) return;
-        uint32_t w = w16[0];
-        uint32_t wn = (nw > 1) ? w16[stride] : 0u;
-        float4 xa = sx[w & 0xffffu], na = sn[w & 0xffffu];
-        for (int m = 0; m < nw; ++m) {
-            const uint32_t wc = w;
-            w = wn;
-            if (m + 2 < nw) wn = w16[(long long)(m + 2) * stride];
-            const uint32_t lb = wc >> 16;
-            const float4 xb = sx[lb], nb = sn[lb];
-            pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xa, na, t, out);
-            if (m + 1 < nw) { xa = sx[w & 0xffffu]; na = sn[w & 0xffffu]; }
-            pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xb, nb, t, out);
-        }
-        return;
-    }
-    constexpr int G = HALO_GROUP, W = G / 2;
+                                           const float4 *sn, const PairRec *spp, int crow, const Anchor &ag,
+                                           const AnchorI &ai, float4 xi, float4 ni, Acc<double> &out) {
     const int nw = halo_row_words(nn);
-    uint32_t wn[W];
-#pragma unroll
-    for (int q = 0; q < W; ++q) wn[q] = (nw > 0) ? w16[(long long)q * stride] : 0u;
-    for (int m = 0; m < nw; m += W) {
-        uint32_t wc[W];
-#pragma unroll
-        for (int q = 0; q < W; ++q) wc[q] = wn[q];
-        if (m + W < nw) {
-#pragma unroll
-            for (int q = 0; q < W; ++q) wn[q] = w16[(long long)(m + W + q) * stride];
-        }
-        float4 xj[G], nj[G];
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-            const uint32_t l = (u & 1) ? (wc[u >> 1] >> 16) : (wc[u >> 1] & 0xffffu);
-            xj[u] = sx[l]; nj[u] = sn[l];
-        }
-        Acc<float> acc32;
-        Acc<PairAcc> &acc = *[&]() {
-            if constexpr (ORBC_PAIR_F64 >= 2) return &out; else return &acc32;
-        }();
-#pragma unroll
-        for (int u = 0; u < G; ++u) pair_mixed<ENERGY>(ag, ai, xi, ni, ti, xj[u], nj[u], t, acc);
-        if constexpr (ORBC_PAIR_F64 < 2) add_to(out, acc32);
+    if (nw == 0) return;
+    uint32_t w = w16[0];
+    uint32_t wn = (nw > 1) ? w16[stride] : 0u;
+    float4 xa = sx[w & 0x3ffu], na = sn[w & 0x3ffu];
+    for (int m = 0; m < nw; ++m) {
+        const uint32_t wc = w;
+        w = wn;
+        if (m + 2 < nw) wn = w16[(long long)(m + 2) * stride];
+        const uint32_t eb = wc >> 16;
+        const float4 xb = sx[eb & 0x3ffu], nb = sn[eb & 0x3ffu];
+        const PairRec pa = spp[crow + (int)((wc >> 10) & 7u)];
+        pair_rec(ag, ai, xi, ni, xa, na, pa, out);
+        if (m + 1 < nw) { xa = sx[w & 0x3ffu]; na = sn[w & 0x3ffu]; }
+        const PairRec pb = spp[crow + (int)(eb >> 10)];
+        pair_rec(ag, ai, xi, ni, xb, nb, pb, out);
     }
 }
 
-// compute_bonded from the FP32 mirrors (f only)
-template <bool ENERGY>
+// compute_bonded from the FP32 mirrors (f only); TAB holds K[] and r0[]
+template <bool ENERGY, typename TAB>
 __device__ __forceinline__ void bonds_mixed(long long i, long long stride, const PQ *pq, const uint32_t *bnd,
                                             const uint8_t *nbnd, const Anchor &ag, const AnchorI &ai, float4 xi,
-                                            const FFS<float> &t, Acc<double> &out) {
+                                            const TAB &t, Acc<double> &out) {
     const int nb = nbnd[i];
     Acc<float> acc;
     for (int k = 0; k < nb; ++k) {
@@ -704,8 +758,20 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         a.dslot[(k3 + 1) % 3] = 0u;
         atomicAdd(a.done, 1);
     }
+    // force-field constants: the FP32/FP64 tables, or (block halos) one
+    // record per type pair and the bond constants
+    // (each instantiation keeps only the LDS arrays it references)
+    struct BondTab { float K[NBT], r0[NBT]; };
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
-    load_ffs(&tab);
+    __shared__ PairRec s_pp[36];
+    __shared__ BondTab s_bt[1];
+    if constexpr (HALO) {
+        load_pair_recs(s_pp);
+        if (threadIdx.x < NBT) { s_bt[0].K[threadIdx.x] = c_fff.bond_K[threadIdx.x]; s_bt[0].r0[threadIdx.x] = c_fff.bond_r0[threadIdx.x]; }
+        __syncthreads();
+    } else {
+        load_ffs(&tab);
+    }
     const long long blk = block_index(a.swizzle);
     const long long i = blk * BLOCK + threadIdx.x;
     const bool act = i < a.n;
@@ -739,12 +805,17 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
             wi = __float_as_uint(xi4.w);
             const AnchorI ai = unpack(wi, a.anc_grid);
             const int nn = (int)a.nnbr[i];
-            if (hcnt != HALO_OVERFLOW)
-                range_halo<false>(nn, a.stride, a.nbr16 + i, s_hx, s_hn, a.anc_grid, ai, xi4, ni4, ti, tab, acc);
-            else
-                range_mixed<false, 1, false>(i, 0, nn, a.stride, a.pq_in, a.nbr + i, a.anc_grid, ai, xi4, ni4, ti,
-                                             tab, acc);
-            if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, a.anc_grid, ai, xi4, tab, acc);
+            const int crow = ti * NT;
+            if (hcnt != HALO_OVERFLOW) {
+                range_halo(nn, a.stride, a.nbr16 + i, s_hx, s_hn, s_pp, crow, a.anc_grid, ai, xi4, ni4, acc);
+            } else {
+                // the block's neighbours did not fit: gather them one at a time
+                for (int k = 0; k < nn; ++k) {
+                    const PQ q = a.pq_in[a.nbr[(long long)k * a.stride + i]];
+                    pair_rec(a.anc_grid, ai, xi4, ni4, q.x, q.n, s_pp[crow + __float_as_int(q.n.w)], acc);
+                }
+            }
+            if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, a.anc_grid, ai, xi4, s_bt[0], acc);
         } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
             const float4 xi4 = qi.x, ni4 = qi.n;

@@ -128,7 +128,7 @@ struct PQ {
 // Block halos of the mixed step (k_halo, k_step_langevin): particle block b
 // (the BLOCK particles from b * BLOCK in store order) stages its own mirrors
 // and those of at most HALO_EXT distinct outside neighbours in LDS, and walks
-// its list as 16-bit indices into that table.  A block with more outside
+// its list as 16-bit entries (10-bit index into that table, 3-bit type).  A block with more outside
 // neighbours is marked HALO_OVERFLOW and gathers from the 32-bit list.
 #ifndef ORBC_HALO_CAP
 #define ORBC_HALO_CAP 896
@@ -275,8 +275,8 @@ void launch_bondlist(long long n, long long stride, const uint32_t *slot, const 
                      uint32_t *bnd, uint8_t *nbnd, hipStream_t s);
 // block halos from the 32-bit list: ext (HALO_EXT per block), cnt (per block),
 // nbr16 (halo_words(kmax) words per particle, column stride `stride`)
-void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, uint32_t *ext,
-                 uint32_t *cnt, uint32_t *nbr16, hipStream_t s);
+void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
+                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, hipStream_t s);
 
 // k_force.hip
 void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool bond, hipStream_t s);
