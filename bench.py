@@ -226,7 +226,10 @@ def side_workload(args):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 forces / fp64 state",
             "data": "synthetic", "config": {"workload": desc, "n_particles": n, "rebuilds": rebuilds,
                                             "temperature": obs["temperature"],
-                                            "lipids_deleted": obs["lipids_deleted"]}}
+                                            "lipids_deleted": obs["lipids_deleted"],
+                                            "step_kernel_ms": tm["step_kernel_ms"] / max(1, tm["step_launches"]),
+                                            "block_halos": {"blocks": obs["halo_blocks"],
+                                                            "overflow": obs["halo_overflow"]}}}
     print(json.dumps(line), flush=True)
 
 

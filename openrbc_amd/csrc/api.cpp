@@ -1197,9 +1197,16 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 cleanup_if_due(c);
                 nkT = 1.5 * (double)c->n * c->cfg.kBT;
                 nb = reduce_blocks(c->n);
-                forces(true);
-                a = integ_args(c);
-                launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
+                if (cells) {
+                    // the volume push joins the forces before the second half
+                    forces(true);
+                    a = integ_args(c);
+                    launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
+                } else {
+                    // forces and the second half in one pass, same arithmetic
+                    launch_step_nh_second(step_args(c), c->mixed(), c->f32, c->t32, c->f64, c->t64, c->red, c->scal,
+                                          c->stream);
+                }
                 launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
                 c->step++;
             }

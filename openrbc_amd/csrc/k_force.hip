@@ -745,6 +745,57 @@ __device__ inline void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, u
 // HALO (mixed only): the block's neighbours are staged in LDS from its halo
 // (k_halo) and the list is walked as 16-bit local indices; a block marked
 // HALO_OVERFLOW gathers from the 32-bit list, one neighbour at a time.
+// ---- block halos in the fused step kernels ----------------------------------
+// LDS of a block-halo step kernel: the halo's mirrors (own particles at
+// [0, BLOCK), outside neighbours after them), one record per type pair, the
+// bond constants.
+struct HaloLDS {
+    float4 x[HALO_CAP], n[HALO_CAP];
+    PairRec pp[36];
+    float K[NBT], r0[NBT];
+};
+
+// Every thread of the block: the constants and, unless the block's halo
+// overflowed (*hcnt = HALO_OVERFLOW), the halo mirrors; returns the mirror of
+// particle i (blk: this block's particle block).
+__device__ __forceinline__ PQ halo_stage(const StepArgs &a, long long blk, bool act, long long i, HaloLDS &L,
+                                         uint32_t &hcnt) {
+    load_pair_recs(L.pp);
+    if (threadIdx.x < NBT) { L.K[threadIdx.x] = c_fff.bond_K[threadIdx.x]; L.r0[threadIdx.x] = c_fff.bond_r0[threadIdx.x]; }
+    hcnt = a.halo_cnt[blk];
+    PQ qi{};
+    if (act) qi = a.pq_in[i];
+    if (hcnt != HALO_OVERFLOW) {
+        L.x[threadIdx.x] = qi.x; L.n[threadIdx.x] = qi.n;
+        const uint32_t *ext = a.halo_ext + blk * HALO_EXT;
+        for (uint32_t q = threadIdx.x; q < hcnt; q += BLOCK) {
+            const PQ p = a.pq_in[ext[q]];
+            L.x[BLOCK + q] = p.x; L.n[BLOCK + q] = p.n;
+        }
+    }
+    __syncthreads();
+    return qi;
+}
+
+// pair and bond forces of active particle i from the staged halo (or, for an
+// overflowed block, gathered one neighbour at a time)
+__device__ __forceinline__ void halo_accumulate(const StepArgs &a, long long i, const PQ &qi, uint32_t hcnt,
+                                                const HaloLDS &L, Acc<double> &acc) {
+    const int ti = __float_as_int(qi.n.w);
+    const AnchorI ai = unpack(__float_as_uint(qi.x.w), a.anc_grid);
+    const int nn = (int)a.nnbr[i];
+    const int crow = ti * NT;
+    if (hcnt != HALO_OVERFLOW) {
+        range_halo(nn, a.stride, a.nbr16 + i, L.x, L.n, L.pp, crow, a.anc_grid, ai, qi.x, qi.n, acc);
+    } else {
+        for (int k = 0; k < nn; ++k) {
+            const PQ q = a.pq_in[a.nbr[(long long)k * a.stride + i]];
+            pair_rec(a.anc_grid, ai, qi.x, qi.n, q.x, q.n, L.pp[crow + __float_as_int(q.n.w)], acc);
+        }
+    }
+    if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, a.anc_grid, ai, qi.x, L, acc);
+}
+
 template <bool MIXED, bool HALO>
 __device__ __forceinline__ void step_langevin(const StepArgs &a) {
     static_assert(MIXED || !HALO, "block halos hold FP32 mirrors");
@@ -758,40 +809,17 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         a.dslot[(k3 + 1) % 3] = 0u;
         atomicAdd(a.done, 1);
     }
-    // force-field constants: the FP32/FP64 tables, or (block halos) one
-    // record per type pair and the bond constants
-    // (each instantiation keeps only the LDS arrays it references)
-    struct BondTab { float K[NBT], r0[NBT]; };
+    // force-field constants: the FP32/FP64 tables, or the block halo with
+    // its constants (each instantiation keeps only the LDS it references)
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
-    __shared__ PairRec s_pp[36];
-    __shared__ BondTab s_bt[1];
-    if constexpr (HALO) {
-        load_pair_recs(s_pp);
-        if (threadIdx.x < NBT) { s_bt[0].K[threadIdx.x] = c_fff.bond_K[threadIdx.x]; s_bt[0].r0[threadIdx.x] = c_fff.bond_r0[threadIdx.x]; }
-        __syncthreads();
-    } else {
-        load_ffs(&tab);
-    }
+    __shared__ HaloLDS hl;
     const long long blk = block_index(a.swizzle);
     const long long i = blk * BLOCK + threadIdx.x;
     const bool act = i < a.n;
-    // block halo: own mirrors at [0, BLOCK), outside neighbours after them
-    __shared__ float4 s_hx[HALO ? HALO_CAP : 1], s_hn[HALO ? HALO_CAP : 1];
     uint32_t hcnt = HALO_OVERFLOW;
     PQ qi{};
-    if constexpr (HALO) {
-        hcnt = a.halo_cnt[blk];
-        if (act) qi = a.pq_in[i];
-        if (hcnt != HALO_OVERFLOW) {
-            s_hx[threadIdx.x] = qi.x; s_hn[threadIdx.x] = qi.n;
-            const uint32_t *ext = a.halo_ext + blk * HALO_EXT;
-            for (uint32_t q = threadIdx.x; q < hcnt; q += BLOCK) {
-                const PQ p = a.pq_in[ext[q]];
-                s_hx[BLOCK + q] = p.x; s_hn[BLOCK + q] = p.n;
-            }
-            __syncthreads();
-        }
-    }
+    if constexpr (HALO) qi = halo_stage(a, blk, act, i, hl, hcnt);
+    else load_ffs(&tab);
     float d2 = 0.f;
     bool bad = false;
     if (act) {
@@ -800,22 +828,9 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         uint32_t wi;
         Acc<double> acc;
         if constexpr (HALO) {
-            const float4 xi4 = qi.x, ni4 = qi.n;
-            ti = __float_as_int(ni4.w);
-            wi = __float_as_uint(xi4.w);
-            const AnchorI ai = unpack(wi, a.anc_grid);
-            const int nn = (int)a.nnbr[i];
-            const int crow = ti * NT;
-            if (hcnt != HALO_OVERFLOW) {
-                range_halo(nn, a.stride, a.nbr16 + i, s_hx, s_hn, s_pp, crow, a.anc_grid, ai, xi4, ni4, acc);
-            } else {
-                // the block's neighbours did not fit: gather them one at a time
-                for (int k = 0; k < nn; ++k) {
-                    const PQ q = a.pq_in[a.nbr[(long long)k * a.stride + i]];
-                    pair_rec(a.anc_grid, ai, xi4, ni4, q.x, q.n, s_pp[crow + __float_as_int(q.n.w)], acc);
-                }
-            }
-            if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, a.anc_grid, ai, xi4, s_bt[0], acc);
+            ti = __float_as_int(qi.n.w);
+            wi = __float_as_uint(qi.x.w);
+            halo_accumulate(a, i, qi, hcnt, hl, acc);
         } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
             const float4 xi4 = qi.x, ni4 = qi.n;
@@ -898,22 +913,33 @@ __global__ __launch_bounds__(BLOCK) ORBC_HALO_ATTR void k_step_langevin_halo(Ste
 }
 
 // ---- fused forces + post_toque_final_update (NH second half) --------------
-// src/integrate_nh.h:237-273 after compute_pairwise_fused/compute_bonded:
-// t = n x t; v += dt/2 (f/m - zeta v); o += dt/2 t; KE partial per block.
-// f and the torque are stored for the next first half.
-template <bool MIXED>
-__global__ __launch_bounds__(BLOCK) void k_step_nh_second(StepArgs a, double *f64, double *t64,
-                                                          double *ke_partial, const double *zeta_p) {
+// src/integrate_nh.h:237-273 after compute_pairwise_fused/compute_bonded, in
+// the arithmetic of the unfused pair k_forces + k_nh_second: the mixed sums
+// are rounded to FP32 as the FP32 force arrays hold them; t = n x t;
+// v += dt/2 (f/m - zeta v); o += dt/2 t; f and n x t stored (FP32 arrays in
+// mixed precision) for the next first half; KE partial per particle block.
+template <bool MIXED, bool HALO>
+__device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, float *t32, double *f64, double *t64,
+                                               double *ke_partial, const double *zeta_p) {
+    static_assert(MIXED || !HALO, "block halos hold FP32 mirrors");
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
-    load_ffs(&tab);
-    const long long i = block_index(a.swizzle) * BLOCK + threadIdx.x;
+    __shared__ HaloLDS hl;
+    const long long blk = block_index(a.swizzle);
+    const long long i = blk * BLOCK + threadIdx.x;
     const bool act = i < a.n;
+    uint32_t hcnt = HALO_OVERFLOW;
+    PQ qh{};
+    if constexpr (HALO) qh = halo_stage(a, blk, act, i, hl, hcnt);
+    else load_ffs(&tab);
     double ke = 0.0;
     if (act) {
-        double fx, fy, fz, tx, ty, tz;
+        double f[3], t[3];
         int ti;
         Acc<double> acc;
-        if constexpr (MIXED) {
+        if constexpr (HALO) {
+            halo_accumulate(a, i, qh, hcnt, hl, acc);
+            ti = __float_as_int(qh.n.w);
+        } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
             accumulate_mixed<false>(i, a.stride, a.pq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid,
                                     qi.x, qi.n, tab, acc, true, a.has_bonds != 0);
@@ -923,21 +949,31 @@ __global__ __launch_bounds__(BLOCK) void k_step_nh_second(StepArgs a, double *f6
                                    a.anc_grid, tab, acc, true, a.has_bonds != 0);
             ti = a.type[i];
         }
-        fx = acc.fx; fy = acc.fy; fz = acc.fz; tx = acc.tx; ty = acc.ty; tz = acc.tz;
-        if (a.f_ext) { fx += a.f_ext[3 * i]; fy += a.f_ext[3 * i + 1]; fz += a.f_ext[3 * i + 2]; }
-        const double zeta = *zeta_p, dt = a.dt;
-        const double nx = a.n_in[3 * i], ny = a.n_in[3 * i + 1], nz = a.n_in[3 * i + 2];
-        const double qx = ny * tz - nz * ty, qy = nz * tx - nx * tz, qz = nx * ty - ny * tx;
-        const double im = c_ffd.inv_mass[ti];
-        double vx = a.v[3 * i], vy = a.v[3 * i + 1], vz = a.v[3 * i + 2];
-        vx += 0.5 * dt * (fx * im - zeta * vx);
-        vy += 0.5 * dt * (fy * im - zeta * vy);
-        vz += 0.5 * dt * (fz * im - zeta * vz);
-        a.v[3 * i] = vx; a.v[3 * i + 1] = vy; a.v[3 * i + 2] = vz;
-        a.o[3 * i] += 0.5 * dt * qx; a.o[3 * i + 1] += 0.5 * dt * qy; a.o[3 * i + 2] += 0.5 * dt * qz;
-        f64[3 * i] = fx; f64[3 * i + 1] = fy; f64[3 * i + 2] = fz;
-        t64[3 * i] = qx; t64[3 * i + 1] = qy; t64[3 * i + 2] = qz;
-        ke = 0.5 * c_ffd.mass[ti] * (vx * vx + vy * vy + vz * vz);
+        if constexpr (MIXED) {
+            f[0] = (float)acc.fx; f[1] = (float)acc.fy; f[2] = (float)acc.fz;
+            t[0] = (float)acc.tx; t[1] = (float)acc.ty; t[2] = (float)acc.tz;
+        } else {
+            f[0] = acc.fx; f[1] = acc.fy; f[2] = acc.fz;
+            t[0] = acc.tx; t[1] = acc.ty; t[2] = acc.tz;
+        }
+        const double m = c_ffd.mass[ti];
+        const double zeta = *zeta_p;
+        double v[3], n[3], q[3];
+        for (int d = 0; d < 3; ++d) { v[d] = a.v[3 * i + d]; n[d] = a.n_in[3 * i + d]; }
+        q[0] = n[1] * t[2] - n[2] * t[1];
+        q[1] = n[2] * t[0] - n[0] * t[2];
+        q[2] = n[0] * t[1] - n[1] * t[0];
+        for (int d = 0; d < 3; ++d) {
+            v[d] += 0.5 * a.dt * (f[d] / m - zeta * v[d]);
+            a.o[3 * i + d] += 0.5 * a.dt * q[d];
+            a.v[3 * i + d] = v[d];
+        }
+        if constexpr (MIXED) {
+            for (int d = 0; d < 3; ++d) { f32[3 * i + d] = (float)f[d]; t32[3 * i + d] = (float)q[d]; }
+        } else {
+            for (int d = 0; d < 3; ++d) { f64[3 * i + d] = f[d]; t64[3 * i + d] = q[d]; }
+        }
+        ke = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ke += __shfl_xor(ke, off, 64);
@@ -947,8 +983,28 @@ __global__ __launch_bounds__(BLOCK) void k_step_nh_second(StepArgs a, double *f6
     if (threadIdx.x == 0) {
         double s = 0;
         for (int w = 0; w < BLOCK / 64; ++w) s += ws[w];
-        ke_partial[blockIdx.x] = s;
+        ke_partial[blk] = s;          // by particle block, the order of the unfused reduction
     }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(BLOCK) void k_step_nh_second(StepArgs a, float *f32, float *t32, double *f64,
+                                                          double *t64, double *ke_partial, const double *zeta_p) {
+    step_nh_second<MIXED, false>(a, f32, t32, f64, t64, ke_partial, zeta_p);
+}
+#ifndef ORBC_NH_HALO_WAVES
+#define ORBC_NH_HALO_WAVES ORBC_HALO_WAVES
+#endif
+#if ORBC_NH_HALO_WAVES > 0
+#define ORBC_NH_HALO_ATTR __attribute__((amdgpu_waves_per_eu(ORBC_NH_HALO_WAVES)))
+#else
+#define ORBC_NH_HALO_ATTR
+#endif
+__global__ __launch_bounds__(BLOCK) ORBC_NH_HALO_ATTR void k_step_nh_second_halo(StepArgs a, float *f32, float *t32,
+                                                                              double *f64, double *t64,
+                                                                              double *ke_partial,
+                                                                              const double *zeta_p) {
+    step_nh_second<true, true>(a, f32, t32, f64, t64, ke_partial, zeta_p);
 }
 
 inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -984,12 +1040,14 @@ void launch_step_langevin(const StepArgs &a, bool mixed, hipStream_t s) {
     else hipLaunchKernelGGL(k_step_langevin<false>, g, b, 0, s, a);
 }
 
-void launch_step_nh_second(const StepArgs &a, bool mixed, double *f64, double *t64, double *ke_partial,
-                           const double *zeta, hipStream_t s) {
+void launch_step_nh_second(const StepArgs &a, bool mixed, float *f32, float *t32, double *f64, double *t64,
+                           double *ke_partial, const double *zeta, hipStream_t s) {
     if (a.n <= 0) return;
     const dim3 g(nblk(a.n)), b(BLOCK);
-    if (mixed) hipLaunchKernelGGL(k_step_nh_second<true>, g, b, 0, s, a, f64, t64, ke_partial, zeta);
-    else hipLaunchKernelGGL(k_step_nh_second<false>, g, b, 0, s, a, f64, t64, ke_partial, zeta);
+    if (mixed && a.halo_cnt)
+        hipLaunchKernelGGL(k_step_nh_second_halo, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
+    else if (mixed) hipLaunchKernelGGL(k_step_nh_second<true>, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
+    else hipLaunchKernelGGL(k_step_nh_second<false>, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
 }
 
 void launch_pairs_export(const ForceArgs &a, bool mixed, unsigned *ids_i, unsigned *ids_j, unsigned long long *count,

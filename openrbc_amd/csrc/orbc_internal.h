@@ -281,8 +281,10 @@ void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint3
 // k_force.hip
 void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool bond, hipStream_t s);
 void launch_step_langevin(const StepArgs &a, bool mixed, hipStream_t s);
-void launch_step_nh_second(const StepArgs &a, bool mixed, double *f64, double *t64, double *ke_partial,
-                           const double *zeta, hipStream_t s);
+// forces + the NH second half in one pass (k_forces + post_toque_final_update);
+// f, n x t go to f32/t32 (mixed) or f64/t64, KE partials per particle block
+void launch_step_nh_second(const StepArgs &a, bool mixed, float *f32, float *t32, double *f64, double *t64,
+                           double *ke_partial, const double *zeta, hipStream_t s);
 long long step_blocks(long long n);
 void launch_pairs_export(const ForceArgs &a, bool mixed, unsigned *ids_i, unsigned *ids_j,
                          unsigned long long *count, long long cap, hipStream_t s);

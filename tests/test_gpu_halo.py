@@ -1,4 +1,4 @@
-"""Block halos of the mixed Langevin step.
+"""Block halos of the mixed fused steps (Langevin, Nose-Hoover second half).
 
 With ``block_halo`` on, each block of 256 particles stages its own FP32
 mirrors and those of its outside neighbours in LDS and walks its list as
@@ -51,7 +51,7 @@ def _with_blob(s, m=16, spacing=1.0, offset=(400.0, 0.0, 0.0), seed=3):
     return out
 
 
-def _run(s, halo, nsteps, minimize=0, **kw):
+def _run(s, halo, nsteps, minimize=0, cells=False, **kw):
     sim = orbc.Simulation(precision=orbc.MIXED, block_halo=halo, **kw)
     t = s["type"]
     lip = np.where(t == 0)[0]
@@ -65,6 +65,8 @@ def _run(s, halo, nsteps, minimize=0, **kw):
         sim.set_bonds(bt, s["id"][bi], s["id"][bj])
     if minimize:
         sim.minimize(minimize)
+    if cells:
+        sim.voronoi_init(0, 4)
     sim.run(nsteps)
     st = {sp: sim.get_state(sp) for sp in (orbc.LIPID, orbc.PROTEIN) if (len(lip) if sp == orbc.LIPID else len(prot))}
     obs = sim.observables()
@@ -83,12 +85,13 @@ def rbc():
     return _rbc(2)
 
 
-@pytest.mark.parametrize("skin", [0.05, 0.4])
-def test_block_halo_run_bit_identical(rbc, skin):
+@pytest.mark.parametrize("skin,thermostat", [(0.05, orbc.RUN_LANGEVIN), (0.4, orbc.RUN_LANGEVIN),
+                                              (0.05, orbc.RUN_NOSE_HOOVER)])
+def test_block_halo_run_bit_identical(rbc, skin, thermostat):
     s = rbc
-    print(f"halo run: icosphere-2 cell, n={len(s['type'])}, skin {skin}", flush=True)
-    on, obs_on = _run(s, 1, 40, minimize=30, seed=5, skin=skin)
-    off, obs_off = _run(s, 0, 40, minimize=30, seed=5, skin=skin)
+    print(f"halo run: icosphere-2 cell, n={len(s['type'])}, skin {skin}, thermostat {thermostat}", flush=True)
+    on, obs_on = _run(s, 1, 40, minimize=30, seed=5, skin=skin, thermostat=thermostat)
+    off, obs_off = _run(s, 0, 40, minimize=30, seed=5, skin=skin, thermostat=thermostat)
     _same(on, off)
     assert obs_on["rebuilds"] == obs_off["rebuilds"] and obs_on["rebuilds"] > 1
     assert obs_on["halo_blocks"] == (len(s["type"]) + 255) // 256
@@ -96,10 +99,26 @@ def test_block_halo_run_bit_identical(rbc, skin):
     assert obs_off["halo_blocks"] == 0
 
 
-def test_block_halo_overflow_blocks_gather_globally(rbc):
+@pytest.mark.parametrize("thermostat", [orbc.RUN_LANGEVIN, orbc.RUN_NOSE_HOOVER])
+def test_block_halo_overflow_blocks_gather_globally(rbc, thermostat):
     s = _with_blob(rbc)
-    print(f"halo overflow run: n={len(s['type'])}", flush=True)
-    on, obs_on = _run(s, 1, 6, minimize=20, seed=4, skin=0.2)
-    off, _ = _run(s, 0, 6, minimize=20, seed=4, skin=0.2)
+    print(f"halo overflow run: n={len(s['type'])}, thermostat {thermostat}", flush=True)
+    on, obs_on = _run(s, 1, 6, minimize=20, seed=4, skin=0.2, thermostat=thermostat)
+    off, _ = _run(s, 0, 6, minimize=20, seed=4, skin=0.2, thermostat=thermostat)
     _same(on, off)
     assert 0 < obs_on["halo_overflow"] < obs_on["halo_blocks"]
+
+
+@pytest.mark.parametrize("halo", [0, 1])
+def test_fused_nose_hoover_matches_unfused(rbc, halo):
+    """The Nose-Hoover second half fused with the forces (no Voronoi cells in
+    the run) against the separate force and integrate kernels (taken while
+    cells are kept for the stray-lipid cleanup, which finds nothing to remove
+    here): the same arithmetic, so the same trajectory bit for bit."""
+    s = rbc
+    kw = dict(minimize=30, seed=5, skin=0.05, thermostat=orbc.RUN_NOSE_HOOVER)
+    fused, obs_f = _run(s, halo, 30, **kw)
+    split, obs_s = _run(s, halo, 30, cells=True, **kw)
+    assert obs_s["lipids_deleted"] == 0
+    assert obs_f["rebuilds"] == obs_s["rebuilds"] and obs_f["rebuilds"] > 1
+    _same(fused, split)
