@@ -374,7 +374,8 @@ def main():
             "unit": "GB/s",
             "frac": head["achieved_GBs"] * 1e9 / HBM_PEAK,
             "traffic": traffic,
-            "kernel": "k_step_langevin (fused pair + bond + Langevin)",
+            "kernel": ("k_step_langevin_halo" if head["halo_blocks"] else "k_step_langevin")
+                      + " (fused pair + bond + Langevin)",
             "kernel_ms": head["kernel_ms"],
             "algorithmic_bytes_per_launch": head["bytes_per_step"],
         },

@@ -96,9 +96,11 @@ typedef struct orbc_config {
     double volume_strength;     /* constrain_volume strength */
     int device;                 /* HIP device ordinal */
     int block_size;             /* force/step kernel block size: 64/128/256/512; 0 = tuned default */
-    int block_halo;             /* mixed Langevin step: 1 (default) stages each particle block's
-                                   neighbours in LDS and walks 16-bit local indices; 0 gathers every
-                                   neighbour from the 32-bit list.  Same forces either way. */
+    int block_halo;             /* mixed fused steps (Langevin, Nose-Hoover second half): stage each
+                                   particle block's neighbours in LDS and walk 16-bit local indices
+                                   (built at every neighbour rebuild).  1 (default): while a list lasts
+                                   4 steps or more on average; 2: always; 0: never (every neighbour
+                                   gathered from the 32-bit list).  Same forces either way. */
 } orbc_config;
 
 typedef struct orbc_observables {

@@ -506,9 +506,13 @@ void rebuild(orbc_ctx *c) {
         c->kmax = (int)mc + 16;
         c->nbr = c->mem.alloc<uint32_t>((size_t)c->kmax * c->stride);
     }
-    // 7b. block halos of the mixed step
+    // 7b. block halos of the mixed step.  Automatic mode builds them only while
+    // a list serves several steps: the build costs about what three steps
+    // save (full RBC: 0.39 ms per build against 0.12 ms per step).
     c->halo_ok = false;
-    if (c->mixed() && c->cfg.block_halo) {
+    const double life = c->cfg.rebuild_period > 0 ? (double)c->cfg.rebuild_period : c->list_life;
+    const bool halo = c->cfg.block_halo == 2 || (c->cfg.block_halo == 1 && (life <= 0.0 || life >= 4.0));
+    if (c->mixed() && halo) {
         const long long nb = (n + BLOCK - 1) / BLOCK;
         const long long words = halo_words(c->kmax) * c->stride;
         if (nb > c->halo_nblk_cap) {
@@ -635,6 +639,7 @@ void orbc_config_default(orbc_config *cfg) {
 
 static bool config_valid(const orbc_config &cfg_) {
     if (cfg_.dt <= 0 || cfg_.skin <= 0 || cfg_.kBT < 0 || cfg_.eta < 0) return false;
+    if (cfg_.block_halo < 0 || cfg_.block_halo > 2) return false;
     if (cfg_.precision != ORBC_FP64 && cfg_.precision != ORBC_MIXED) return false;
     for (int d = 0; d < 3; ++d)
         if (cfg_.periodic[d] && !(cfg_.box_hi[d] > cfg_.box_lo[d])) return false;

@@ -51,8 +51,9 @@ def _with_blob(s, m=16, spacing=1.0, offset=(400.0, 0.0, 0.0), seed=3):
     return out
 
 
-def _run(s, halo, nsteps, minimize=0, cells=False, **kw):
-    sim = orbc.Simulation(precision=orbc.MIXED, block_halo=halo, **kw)
+def _run(s, halo, nsteps, minimize=0, cells=False, mode=None, **kw):
+    # block_halo 2: halos at every rebuild; 0: never; 1: while lists last 4+ steps
+    sim = orbc.Simulation(precision=orbc.MIXED, block_halo=2 * halo if mode is None else mode, **kw)
     t = s["type"]
     lip = np.where(t == 0)[0]
     prot = np.where(t != 0)[0]
@@ -122,3 +123,14 @@ def test_fused_nose_hoover_matches_unfused(rbc, halo):
     assert obs_s["lipids_deleted"] == 0
     assert obs_f["rebuilds"] == obs_s["rebuilds"] and obs_f["rebuilds"] > 1
     _same(fused, split)
+
+
+def test_block_halo_automatic_mode_skips_short_lived_lists(rbc):
+    """Lists that last a step or two (tiny skin): the automatic mode stops
+    building halos after the first rebuilds; the trajectory is unchanged."""
+    s = rbc
+    auto, obs_a = _run(s, 0, 30, minimize=30, seed=5, skin=0.01, mode=1)
+    off, obs_o = _run(s, 0, 30, minimize=30, seed=5, skin=0.01, mode=0)
+    _same(auto, off)
+    assert obs_a["rebuilds"] > 10
+    assert obs_a["halo_blocks"] == 0
