@@ -411,24 +411,26 @@ __device__ inline void block_sum_store(double v, double *partial) {
     }
 }
 
-// per-cell normal, orientation and cone volume, src/constrain_volume.h:54-62.
-// The centre of the centroids is reduced by every block from the per-block
-// partials of k_centroid_center (3 x ncp); the cell sums are cleared after
+// centre of the centroids from the block partials of k_centroid_center, once
+// per centroid update (one wavefront; fixed summation order)
+__global__ void k_center_finish(const double *center_partial, long long ncp, long long m, double *center, Gate gt) {
+    if (gate_closed(gt)) return;
+    for (int d = 0; d < 3; ++d) {
+        double v = 0;
+        for (long long b = threadIdx.x; b < ncp; b += 64) v += center_partial[d * ncp + b];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (threadIdx.x == 0) center[d] = v / (double)m;
+    }
+}
+
+// per-cell normal, orientation and cone volume, src/constrain_volume.h:54-62,
+// about the centre from k_center_finish; the cell sums are cleared after
 // reading (see k_centroid_finish).  Volume partials go to vol_partial.
 __global__ __launch_bounds__(BLOCK) void k_volume_finish(unsigned long long *sums, const float *c, long long m,
-                                                         const double *center_partial, long long ncp,
-                                                         double *normals, double *vol_partial, Gate gt) {
+                                                         const double *center, double *normals,
+                                                         double *vol_partial, Gate gt) {
     if (gate_closed(gt)) return;
-    __shared__ double s_center[3];
-    if (threadIdx.x < 64) {
-        for (int d = 0; d < 3; ++d) {
-            double v = 0;
-            for (long long b = threadIdx.x; b < ncp; b += 64) v += center_partial[d * ncp + b];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (threadIdx.x == 0) s_center[d] = v / (double)m;
-        }
-    }
-    __syncthreads();
+    const double s_center[3] = {center[0], center[1], center[2]};
     const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
     double vol = 0.0;
     if (k < m) {
@@ -598,20 +600,24 @@ void launch_centroid_finish(const VoroArgs &a, unsigned long long *sums, hipStre
 
 // centre-of-centroids partials (3 x nblk(ncell)) at partial; rerun whenever the centroids change
 void launch_centroid_center(const VoroArgs &a, double *partial, hipStream_t s) {
-    if (a.ncell > 0)
-        hipLaunchKernelGGL(k_centroid_center, dim3((unsigned)nblk(a.ncell)), dim3(BLOCK), 0, s, a.centroids, a.ncell,
+    if (a.ncell > 0) {
+        const long long nbc = nblk(a.ncell);
+        hipLaunchKernelGGL(k_centroid_center, dim3((unsigned)nbc), dim3(BLOCK), 0, s, a.centroids, a.ncell,
                            partial, a.gate);
+        hipLaunchKernelGGL(k_center_finish, dim3(1), dim3(64), 0, s, partial, nbc, a.ncell, partial + 4 * nbc, a.gate);
+    }
 }
 
 void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals, double *partial, double *scal,
                    double target, double strength, hipStream_t s) {
-    // scal: [0] V, [1] push; partial: centre partials (launch_centroid_center), then volume partials
+    // scal: [0] V, [1] push; partial: centre partials (launch_centroid_center), then volume
+    // partials, then the centre (3 doubles at 4 * nbc)
     const long long nbc = nblk(a.ncell);
     if (a.n > 0)
         hipLaunchKernelGGL(k_cell_sums<true>, dim3((unsigned)((a.n + SUM_CHUNK - 1) / SUM_CHUNK)), dim3(BLOCK), 0, s,
                            a, sums);
-    hipLaunchKernelGGL(k_volume_finish, dim3((unsigned)nbc), dim3(BLOCK), 0, s, sums, a.centroids, a.ncell, partial,
-                       nbc, normals, partial + 3 * nbc, a.gate);
+    hipLaunchKernelGGL(k_volume_finish, dim3((unsigned)nbc), dim3(BLOCK), 0, s, sums, a.centroids, a.ncell,
+                       partial + 4 * nbc, normals, partial + 3 * nbc, a.gate);
     hipLaunchKernelGGL(k_volume_force, dim3(1), dim3(64), 0, s, partial + 3 * nbc, nbc, target, strength, scal,
                        a.gate);
 }

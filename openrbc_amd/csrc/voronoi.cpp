@@ -126,6 +126,7 @@ void voro_alloc(orbc_ctx *c, long long ncell) {
     V.normals = c->mem.alloc<double>(3 * ncell);
     V.cref = c->mem.alloc<float>(3 * ncell);
     V.cdisp = c->mem.alloc<unsigned>(1);
+    // centre partials (3 per block), volume partials (1 per block), the centre (3)
     V.partial = c->mem.alloc<double>(4 * (ncell / BLOCK + 2) + 16);
     V.ck = c->mem.alloc<uint32_t>(ncell); V.cv = c->mem.alloc<uint32_t>(ncell);
     V.ck2 = c->mem.alloc<uint32_t>(ncell); V.cv2 = c->mem.alloc<uint32_t>(ncell);
