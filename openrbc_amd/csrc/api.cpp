@@ -1193,9 +1193,13 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
             }
             for (long long s = 0; s < nsteps; ++s) {
                 a = integ_args(c);
+                // the first half also measures the displacement since the build
+                // (the skin check ensure_list reads), so no separate pass runs
+                zero_flag(c, 0);
+                a.rb = c->rb; a.disp_max = c->flags;
                 launch_integrate(ORBC_NH_INITIAL_BOUNCE_CLEAR, a, c->mixed(), c->stream);
                 launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
-                c->moved = true;
+                c->moved = false;
                 c->since_build++;
                 bool rb = c->cfg.rebuild_period > 0 && c->since_build >= c->cfg.rebuild_period;
                 if (rb) { rebuild(c); } else { ensure_list(c); }

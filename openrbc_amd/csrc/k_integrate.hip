@@ -130,6 +130,7 @@ template <typename F>
 __global__ __launch_bounds__(BLOCK) void k_nh_first(IntegArgs a) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     double ke = 0.0;
+    float d2 = 0.f;
     if (i < a.n) {
         const int ty = a.type[i];
         const double m = c_mass[ty];
@@ -153,6 +154,23 @@ __global__ __launch_bounds__(BLOCK) void k_nh_first(IntegArgs a) {
         const double z[3] = {0, 0, 0};
         store_ft<F>(a, i, z, z);
         write_mirror(a, i, x, n, ty);
+        if (a.disp_max) {
+            const uint32_t w = a.anc[i];
+            const Anchor &g = a.anc_grid;
+            const long long A[3] = {(long long)(w & g.mx), (long long)((w >> g.shy) & g.my), (long long)(w >> g.shz)};
+            for (int d = 0; d < 3; ++d) {
+                const float e = __double2float_rn(x[d] - anchor_pos(g, d, A[d])) - a.rb[3 * i + d];
+                d2 += e * e;
+            }
+            if (!isfinite(d2)) d2 = 3.0e38f;
+        }
+    }
+    if (a.disp_max) {
+        // one atomic per wave, only where the wave raises the maximum (as k_disp)
+        for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off, 64));
+        if (__lane_id() == 0 && d2 > 0.f &&
+            __float_as_uint(d2) > __hip_atomic_load(a.disp_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(a.disp_max, __float_as_uint(d2));
     }
     block_sum_store(ke, a.ke_partial);
 }

@@ -309,6 +309,11 @@ struct IntegArgs {
     const double *zeta;       // device scalar
     double *ke_partial;       // per block
     double dr_opt, dn_opt;
+    // NH first half: when disp_max is set, the largest squared displacement of
+    // the new positions from the build-time offsets rb (as k_disp computes it)
+    // is folded into *disp_max (float bits, atomicMax)
+    const float *rb;
+    unsigned *disp_max;
 };
 void launch_integrate(int kernel, const IntegArgs &a, bool mixed, hipStream_t s);
 void launch_minimize(const IntegArgs &a, bool mixed, hipStream_t s);

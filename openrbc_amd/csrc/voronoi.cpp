@@ -18,7 +18,10 @@ using namespace orbc;
 
 namespace {
 
-constexpr double kCentroidCell = 6.0;   // centroid-grid edge (~1.6 x centroid spacing)
+#ifndef ORBC_CENTROID_CELL
+#define ORBC_CENTROID_CELL 6.0
+#endif
+constexpr double kCentroidCell = ORBC_CENTROID_CELL;   // centroid-grid edge (~1.6 x centroid spacing)
 
 int ceil_log2(long long v) {
     int b = 0;
