@@ -96,8 +96,9 @@ typedef struct orbc_config {
     double volume_strength;     /* constrain_volume strength */
     int device;                 /* HIP device ordinal */
     int block_size;             /* force/step kernel block size: 64/128/256/512; 0 = tuned default */
-    int block_halo;             /* mixed fused steps (Langevin, Nose-Hoover second half): stage each
-                                   particle block's neighbours in LDS and walk 16-bit local indices
+    int block_halo;             /* fused steps (Langevin, Nose-Hoover second half): stage each particle
+                                   block's neighbours in LDS (FP32 mirrors, or FP64 positions and
+                                   directors) and walk 16-bit local indices
                                    (built at every neighbour rebuild).  1 (default): while a list lasts
                                    4 steps or more on average; 2: always; 0: never (every neighbour
                                    gathered from the 32-bit list).  Same forces either way. */
@@ -121,8 +122,8 @@ typedef struct orbc_observables {
     int anchor_bits[3];
     long long n_lipid, n_protein;   /* species counts (lipids shrink with the stray-lipid cleanup) */
     long long lipids_deleted;       /* lipids removed by the cleanup so far */
-    long long halo_blocks;          /* particle blocks of the mixed step with an LDS halo at the last
-                                       rebuild (0 when block_halo is off or in FP64) */
+    long long halo_blocks;          /* particle blocks of the fused steps with an LDS halo at the last
+                                       rebuild (0 when no halos were built) */
     long long halo_overflow;        /* blocks whose neighbours did not fit (32-bit gathers) */
 } orbc_observables;
 

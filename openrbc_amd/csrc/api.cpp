@@ -506,13 +506,13 @@ void rebuild(orbc_ctx *c) {
         c->kmax = (int)mc + 16;
         c->nbr = c->mem.alloc<uint32_t>((size_t)c->kmax * c->stride);
     }
-    // 7b. block halos of the mixed step.  Automatic mode builds them only while
+    // 7b. block halos of the fused steps.  Automatic mode builds them only while
     // a list serves several steps: the build costs about what three steps
     // save (full RBC: 0.39 ms per build against 0.12 ms per step).
     c->halo_ok = false;
     const double life = c->cfg.rebuild_period > 0 ? (double)c->cfg.rebuild_period : c->list_life;
     const bool halo = c->cfg.block_halo == 2 || (c->cfg.block_halo == 1 && (life <= 0.0 || life >= 4.0));
-    if (c->mixed() && halo) {
+    if (halo) {
         const long long nb = (n + BLOCK - 1) / BLOCK;
         const long long words = halo_words(c->kmax) * c->stride;
         if (nb > c->halo_nblk_cap) {
@@ -979,7 +979,7 @@ static StepArgs step_args(orbc_ctx *c) {
     a.pq_out = c->pq[out];
     a.type = c->type; a.id = c->id; a.anc = c->anc; a.rb = c->rb;
     a.nbr = c->nbr; a.nnbr = c->nnbr;
-    if (c->halo_ok && c->mixed()) { a.halo_ext = c->halo_ext; a.halo_cnt = c->halo_cnt; a.nbr16 = c->nbr16; }
+    if (c->halo_ok) { a.halo_ext = c->halo_ext; a.halo_cnt = c->halo_cnt; a.nbr16 = c->nbr16; }
     a.bnd = c->nbonds ? c->bnd : nullptr;
     a.nbnd = c->nbonds ? c->nbnd : nullptr;
     a.has_bonds = c->nbonds > 0;

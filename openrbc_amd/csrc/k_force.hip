@@ -796,9 +796,74 @@ __device__ __forceinline__ void halo_accumulate(const StepArgs &a, long long i, 
     if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, a.anc_grid, ai, qi.x, L, acc);
 }
 
+// FP64 block halo: positions and directors of the halo particles (48 bytes,
+// three 16-byte LDS words each); the pair terms go through pair_fp64 with the
+// FP64 tables, in the order of range_fp64, so the forces are those of the
+// gather walk.
+struct alignas(16) XN64 {
+    double x[3], n[3];
+};
+struct HaloLDS64 {
+    XN64 r[HALO_CAP];
+};
+
+__device__ __forceinline__ void halo_stage64(const StepArgs &a, long long blk, bool act, long long i,
+                                             HaloLDS64 &L, uint32_t &hcnt) {
+    hcnt = a.halo_cnt[blk];
+    if (hcnt != HALO_OVERFLOW) {
+        XN64 q{};
+        if (act)
+            for (int d = 0; d < 3; ++d) { q.x[d] = a.x_in[3 * i + d]; q.n[d] = a.n_in[3 * i + d]; }
+        L.r[threadIdx.x] = q;
+        const uint32_t *ext = a.halo_ext + blk * HALO_EXT;
+        for (uint32_t e = threadIdx.x; e < hcnt; e += BLOCK) {
+            const long long j = ext[e];
+            XN64 p;
+            for (int d = 0; d < 3; ++d) { p.x[d] = a.x_in[3 * j + d]; p.n[d] = a.n_in[3 * j + d]; }
+            L.r[BLOCK + e] = p;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void halo_accumulate64(const StepArgs &a, long long i, uint32_t hcnt,
+                                                  const HaloLDS64 &L, const FFS<double> &t, Acc<double> &acc) {
+    const double xi = a.x_in[3 * i], yi = a.x_in[3 * i + 1], zi = a.x_in[3 * i + 2];
+    const double nix = a.n_in[3 * i], niy = a.n_in[3 * i + 1], niz = a.n_in[3 * i + 2];
+    const int ti = a.type[i];
+    const Anchor &ag = a.anc_grid;
+    const bool periodic = ag.L[0] > 0 || ag.L[1] > 0 || ag.L[2] > 0;
+    const int nn = (int)a.nnbr[i];
+    if (hcnt != HALO_OVERFLOW) {
+        const uint32_t *w16 = a.nbr16 + i;
+        const int nw = halo_row_words(nn);
+        for (int m = 0; m < nw; ++m) {
+            const uint32_t w = w16[(long long)m * a.stride];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t e = h ? (w >> 16) : (w & 0xffffu);
+                const XN64 &q = L.r[e & 0x3ffu];
+                pair_fp64<false>(ag, periodic, xi, yi, zi, nix, niy, niz, ti, q.x, q.n, (int)(e >> 10), t, acc);
+            }
+        }
+    } else {
+        range_fp64<false>(i, 0, nn, a.stride, a.x_in, a.n_in, a.type, a.nbr + i, ag, periodic, xi, yi, zi, nix, niy,
+                          niz, ti, t, acc);
+    }
+    if (a.has_bonds && a.nbnd) {
+        const int nb = a.nbnd[i];
+        for (int k = 0; k < nb; ++k) {
+            const uint32_t e = a.bnd[(long long)k * a.stride + i];
+            const long long j = e & BOND_INDEX_MASK;
+            double dx = a.x_in[3 * j] - xi, dy = a.x_in[3 * j + 1] - yi, dz = a.x_in[3 * j + 2] - zi;
+            min_image(ag, dx, dy, dz);
+            bond_ij<double, false>(t, (int)(e >> 30), dx, dy, dz, acc);
+        }
+    }
+}
+
 template <bool MIXED, bool HALO>
 __device__ __forceinline__ void step_langevin(const StepArgs &a) {
-    static_assert(MIXED || !HALO, "block halos hold FP32 mirrors");
     const int k3 = (int)(a.step % 3ull);
     // skin flag of this step's input positions; no block of this launch writes it
     if (*a.stop || a.dslot[(k3 + 2) % 3] != 0u) {
@@ -813,13 +878,18 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
     // its constants (each instantiation keeps only the LDS it references)
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
     __shared__ HaloLDS hl;
+    __shared__ HaloLDS64 hl64;
     const long long blk = block_index(a.swizzle);
     const long long i = blk * BLOCK + threadIdx.x;
     const bool act = i < a.n;
     uint32_t hcnt = HALO_OVERFLOW;
     PQ qi{};
-    if constexpr (HALO) qi = halo_stage(a, blk, act, i, hl, hcnt);
-    else load_ffs(&tab);
+    if constexpr (HALO && MIXED) {
+        qi = halo_stage(a, blk, act, i, hl, hcnt);
+    } else {
+        load_ffs(&tab);
+        if constexpr (HALO) halo_stage64(a, blk, act, i, hl64, hcnt);
+    }
     float d2 = 0.f;
     bool bad = false;
     if (act) {
@@ -827,10 +897,14 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         int ti;
         uint32_t wi;
         Acc<double> acc;
-        if constexpr (HALO) {
+        if constexpr (HALO && MIXED) {
             ti = __float_as_int(qi.n.w);
             wi = __float_as_uint(qi.x.w);
             halo_accumulate(a, i, qi, hcnt, hl, acc);
+        } else if constexpr (HALO) {
+            halo_accumulate64(a, i, hcnt, hl64, tab, acc);
+            ti = a.type[i];
+            wi = a.anc[i];
         } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
             const float4 xi4 = qi.x, ni4 = qi.n;
@@ -911,6 +985,8 @@ __global__ __launch_bounds__(BLOCK) ORBC_STEP_ATTR void k_step_langevin(StepArgs
 __global__ __launch_bounds__(BLOCK) ORBC_HALO_ATTR void k_step_langevin_halo(StepArgs a) {
     step_langevin<true, true>(a);
 }
+// FP64 halo: 48-byte records, 3 blocks of 256 per CU's LDS
+__global__ __launch_bounds__(BLOCK) void k_step_langevin64_halo(StepArgs a) { step_langevin<false, true>(a); }
 
 // ---- fused forces + post_toque_final_update (NH second half) --------------
 // src/integrate_nh.h:237-273 after compute_pairwise_fused/compute_bonded, in
@@ -921,24 +997,31 @@ __global__ __launch_bounds__(BLOCK) ORBC_HALO_ATTR void k_step_langevin_halo(Ste
 template <bool MIXED, bool HALO>
 __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, float *t32, double *f64, double *t64,
                                                double *ke_partial, const double *zeta_p) {
-    static_assert(MIXED || !HALO, "block halos hold FP32 mirrors");
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
     __shared__ HaloLDS hl;
+    __shared__ HaloLDS64 hl64;
     const long long blk = block_index(a.swizzle);
     const long long i = blk * BLOCK + threadIdx.x;
     const bool act = i < a.n;
     uint32_t hcnt = HALO_OVERFLOW;
     PQ qh{};
-    if constexpr (HALO) qh = halo_stage(a, blk, act, i, hl, hcnt);
-    else load_ffs(&tab);
+    if constexpr (HALO && MIXED) {
+        qh = halo_stage(a, blk, act, i, hl, hcnt);
+    } else {
+        load_ffs(&tab);
+        if constexpr (HALO) halo_stage64(a, blk, act, i, hl64, hcnt);
+    }
     double ke = 0.0;
     if (act) {
         double f[3], t[3];
         int ti;
         Acc<double> acc;
-        if constexpr (HALO) {
+        if constexpr (HALO && MIXED) {
             halo_accumulate(a, i, qh, hcnt, hl, acc);
             ti = __float_as_int(qh.n.w);
+        } else if constexpr (HALO) {
+            halo_accumulate64(a, i, hcnt, hl64, tab, acc);
+            ti = a.type[i];
         } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
             accumulate_mixed<false>(i, a.stride, a.pq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid,
@@ -1006,6 +1089,11 @@ __global__ __launch_bounds__(BLOCK) ORBC_NH_HALO_ATTR void k_step_nh_second_halo
                                                                               const double *zeta_p) {
     step_nh_second<true, true>(a, f32, t32, f64, t64, ke_partial, zeta_p);
 }
+__global__ __launch_bounds__(BLOCK) void k_step_nh_second64_halo(StepArgs a, float *f32, float *t32, double *f64,
+                                                                 double *t64, double *ke_partial,
+                                                                 const double *zeta_p) {
+    step_nh_second<false, true>(a, f32, t32, f64, t64, ke_partial, zeta_p);
+}
 
 inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
@@ -1037,6 +1125,7 @@ void launch_step_langevin(const StepArgs &a, bool mixed, hipStream_t s) {
     const dim3 g(nblk(a.n)), b(BLOCK);
     if (mixed && a.halo_cnt) hipLaunchKernelGGL(k_step_langevin_halo, g, b, 0, s, a);
     else if (mixed) hipLaunchKernelGGL(k_step_langevin<true>, g, b, 0, s, a);
+    else if (a.halo_cnt) hipLaunchKernelGGL(k_step_langevin64_halo, g, b, 0, s, a);
     else hipLaunchKernelGGL(k_step_langevin<false>, g, b, 0, s, a);
 }
 
@@ -1047,6 +1136,8 @@ void launch_step_nh_second(const StepArgs &a, bool mixed, float *f32, float *t32
     if (mixed && a.halo_cnt)
         hipLaunchKernelGGL(k_step_nh_second_halo, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
     else if (mixed) hipLaunchKernelGGL(k_step_nh_second<true>, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
+    else if (a.halo_cnt)
+        hipLaunchKernelGGL(k_step_nh_second64_halo, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
     else hipLaunchKernelGGL(k_step_nh_second<false>, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
 }
 

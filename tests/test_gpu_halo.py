@@ -1,11 +1,12 @@
-"""Block halos of the mixed fused steps (Langevin, Nose-Hoover second half).
+"""Block halos of the fused steps (Langevin, Nose-Hoover second half).
 
-With ``block_halo`` on, each block of 256 particles stages its own FP32
-mirrors and those of its outside neighbours in LDS and walks its list as
-16-bit local indices; a block whose outside neighbours do not fit gathers from
-the 32-bit list.  The pair terms and their order are the same either way, so a
-run with halos must be bit-identical to a run without them, through neighbour
-rebuilds, with and without blocks that overflow.
+With ``block_halo`` on, each block of 256 particles stages its own data (FP32
+mirrors, or FP64 positions and directors) and that of its outside neighbours
+in LDS and walks its list as 16-bit local indices; a block whose outside
+neighbours do not fit gathers from the 32-bit list.  The pair terms and their
+order are the same either way, so a run with halos must be bit-identical to a
+run without them, through neighbour rebuilds, with and without blocks that
+overflow.
 """
 import numpy as np
 import pytest
@@ -51,9 +52,9 @@ def _with_blob(s, m=16, spacing=1.0, offset=(400.0, 0.0, 0.0), seed=3):
     return out
 
 
-def _run(s, halo, nsteps, minimize=0, cells=False, mode=None, **kw):
+def _run(s, halo, nsteps, minimize=0, cells=False, mode=None, precision=orbc.MIXED, **kw):
     # block_halo 2: halos at every rebuild; 0: never; 1: while lists last 4+ steps
-    sim = orbc.Simulation(precision=orbc.MIXED, block_halo=2 * halo if mode is None else mode, **kw)
+    sim = orbc.Simulation(precision=precision, block_halo=2 * halo if mode is None else mode, **kw)
     t = s["type"]
     lip = np.where(t == 0)[0]
     prot = np.where(t != 0)[0]
@@ -86,13 +87,16 @@ def rbc():
     return _rbc(2)
 
 
+@pytest.mark.parametrize("precision", [orbc.MIXED, orbc.FP64])
 @pytest.mark.parametrize("skin,thermostat", [(0.05, orbc.RUN_LANGEVIN), (0.4, orbc.RUN_LANGEVIN),
                                               (0.05, orbc.RUN_NOSE_HOOVER)])
-def test_block_halo_run_bit_identical(rbc, skin, thermostat):
+def test_block_halo_run_bit_identical(rbc, skin, thermostat, precision):
     s = rbc
-    print(f"halo run: icosphere-2 cell, n={len(s['type'])}, skin {skin}, thermostat {thermostat}", flush=True)
-    on, obs_on = _run(s, 1, 40, minimize=30, seed=5, skin=skin, thermostat=thermostat)
-    off, obs_off = _run(s, 0, 40, minimize=30, seed=5, skin=skin, thermostat=thermostat)
+    print(f"halo run: icosphere-2 cell, n={len(s['type'])}, skin {skin}, thermostat {thermostat}, "
+          f"precision {precision}", flush=True)
+    kw = dict(minimize=30, seed=5, skin=skin, thermostat=thermostat, precision=precision)
+    on, obs_on = _run(s, 1, 40, **kw)
+    off, obs_off = _run(s, 0, 40, **kw)
     _same(on, off)
     assert obs_on["rebuilds"] == obs_off["rebuilds"] and obs_on["rebuilds"] > 1
     assert obs_on["halo_blocks"] == (len(s["type"]) + 255) // 256
@@ -100,12 +104,14 @@ def test_block_halo_run_bit_identical(rbc, skin, thermostat):
     assert obs_off["halo_blocks"] == 0
 
 
+@pytest.mark.parametrize("precision", [orbc.MIXED, orbc.FP64])
 @pytest.mark.parametrize("thermostat", [orbc.RUN_LANGEVIN, orbc.RUN_NOSE_HOOVER])
-def test_block_halo_overflow_blocks_gather_globally(rbc, thermostat):
+def test_block_halo_overflow_blocks_gather_globally(rbc, thermostat, precision):
     s = _with_blob(rbc)
-    print(f"halo overflow run: n={len(s['type'])}, thermostat {thermostat}", flush=True)
-    on, obs_on = _run(s, 1, 6, minimize=20, seed=4, skin=0.2, thermostat=thermostat)
-    off, _ = _run(s, 0, 6, minimize=20, seed=4, skin=0.2, thermostat=thermostat)
+    print(f"halo overflow run: n={len(s['type'])}, thermostat {thermostat}, precision {precision}", flush=True)
+    kw = dict(minimize=20, seed=4, skin=0.2, thermostat=thermostat, precision=precision)
+    on, obs_on = _run(s, 1, 6, **kw)
+    off, _ = _run(s, 0, 6, **kw)
     _same(on, off)
     assert 0 < obs_on["halo_overflow"] < obs_on["halo_blocks"]
 
