@@ -799,7 +799,10 @@ __device__ __forceinline__ void halo_accumulate(const StepArgs &a, long long i, 
 // FP64 block halo: positions and directors of the halo particles (48 bytes,
 // three 16-byte LDS words each); the pair terms go through pair_fp64 with the
 // FP64 tables, in the order of range_fp64, so the forces are those of the
-// gather walk.
+// gather walk.  The FP64 terms are written as plain expressions whose
+// multiply-adds the compiler fuses; a walk that interleaves the loads of the
+// next entry with a pair's arithmetic (measured) changes which ones it fuses,
+// so this walk stays as simple as range_fp64's.
 struct alignas(16) XN64 {
     double x[3], n[3];
 };
