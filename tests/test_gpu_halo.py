@@ -140,3 +140,20 @@ def test_block_halo_automatic_mode_skips_short_lived_lists(rbc):
     _same(auto, off)
     assert obs_a["rebuilds"] > 10
     assert obs_a["halo_blocks"] == 0
+
+
+@pytest.mark.parametrize("precision", [orbc.MIXED, orbc.FP64])
+@pytest.mark.parametrize("thermostat", [orbc.RUN_LANGEVIN, orbc.RUN_NOSE_HOOVER])
+def test_block_halo_periodic_box_bit_identical(precision, thermostat):
+    """Minimum images on the periodic axes: the halo walks take the same
+    anchor-difference wrap (mixed) and FP64 minimum image as the gather walks."""
+    from _systems import periodic_patch
+    s = periodic_patch(48, 48, seed=3)
+    lo, hi, per = s["box"]
+    print(f"halo periodic run: n={len(s['type'])}, precision {precision}, thermostat {thermostat}", flush=True)
+    kw = dict(minimize=20, seed=6, skin=0.1, thermostat=thermostat, precision=precision,
+              box_lo=list(lo), box_hi=list(hi), periodic=[int(p) for p in per])
+    on, obs_on = _run(s, 1, 30, **kw)
+    off, _ = _run(s, 0, 30, **kw)
+    _same(on, off)
+    assert obs_on["halo_blocks"] == (len(s["type"]) + 255) // 256 and obs_on["rebuilds"] > 1
