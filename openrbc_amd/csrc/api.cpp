@@ -499,7 +499,10 @@ void rebuild(orbc_ctx *c) {
             a.aff = c->voro.aff; a.centroids = c->voro.c;
         }
         zero_flag(c, 2);
-        launch_nlist(a, c->keys, nocc, c->far_rings, s);
+        // scratch of the outer-shell pass: the cell-list flags and positions are
+        // spent, and the sort values after the permutation; flags[7] counts
+        const FarScratch fs{c->keys_alt, c->vals_alt, c->vals, c->flags + 7, c->sort_temp};
+        launch_nlist(a, c->keys, nocc, c->far_rings, fs, s);
         const unsigned mc = read_flag(c, 2);
         if ((int)mc <= c->kmax) break;
         c->mem.release(c->nbr);

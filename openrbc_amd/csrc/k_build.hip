@@ -9,6 +9,8 @@
 // partners within rc(ti,tj) + skin, so the force kernel sees every pair the
 // cutoff admits (the reference's centroid stencil can miss some; the
 // Voronoi filter below reproduces its set when asked to).
+#include <algorithm>
+
 #include "orbc_internal.h"
 
 namespace orbc {
@@ -516,7 +518,23 @@ ORBC_PRAGMA(unroll ORBC_NL_UNROLL)
 // past the first ring of cells, appended after the k_nlist entries.  One
 // wavefront per 64 particles; each such particle is handled by the whole wave
 // (nl_wave_shell).  nnbr holds the first-ring count on entry.
-__global__ __launch_bounds__(BLOCK) void k_nlist_far(NlistArgs a) {
+// the particles whose type searches past the first ring, in store order
+__global__ __launch_bounds__(BLOCK) void k_far_flags(long long n, const uint8_t *type, uint32_t *flag) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) flag[i] = c_rings[type[i]] >= 2 ? 1u : 0u;
+}
+__global__ __launch_bounds__(BLOCK) void k_far_compact(long long n, const uint32_t *flag, const uint32_t *pos,
+                                                       uint32_t *list, uint32_t *count) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (flag[i]) list[pos[i]] = (uint32_t)i;
+    if (i == n - 1) *count = pos[i] + flag[i];
+}
+
+// list[0, *count): the particles to extend; the waves of the grid take them in
+// turn (these proteins cluster in the store, so one wave per 64 consecutive
+// particles would serialise them on a few waves)
+__global__ __launch_bounds__(BLOCK) void k_nlist_far(NlistArgs a, const uint32_t *list, const uint32_t *count) {
     __shared__ float4 s_x[BLOCK / 64][NL_CAP];
     __shared__ uint32_t s_j[BLOCK / 64][NL_CAP];
     __shared__ uint32_t s_pref2[BLOCK / 64][65];
@@ -525,15 +543,12 @@ __global__ __launch_bounds__(BLOCK) void k_nlist_far(NlistArgs a) {
     if (threadIdx.x < 36) s_cut2[threadIdx.x] = c_list_cut2[threadIdx.x];
     __syncthreads();
     const int w = threadIdx.x >> 6, lane = __lane_id();
-    const long long base = ((long long)blockIdx.x * (BLOCK / 64) + w) * 64;
-    const long long il = base + lane;
-    unsigned long long far = __ballot(il < a.n && c_rings[a.type[il]] >= 2);
+    const long long m = *count;
+    const long long nw = (long long)gridDim.x * (BLOCK / 64);
     const Grid &g = a.g;
     const Anchor &an = a.anc;
-    while (far) {
-        const int src = __ffsll((long long)far) - 1;
-        far &= far - 1;
-        const long long i = base + src;              // wave-uniform
+    for (long long q = (long long)blockIdx.x * (BLOCK / 64) + w; q < m; q += nw) {
+        const long long i = list[q];                 // wave-uniform
         NlCtx c{a, i, (int)a.type[i], a.pq[i].x, 0, 0, 0, 0.f, 0.f, 0.f, (int)a.nnbr[i]};
         const uint32_t wi = __float_as_uint(c.xi.w);
         c.ax = (int)(wi & an.mx); c.ay = (int)((wi >> an.shy) & an.my); c.az = (int)(wi >> an.shz);
@@ -784,11 +799,19 @@ long long launch_cell_list(const double *x, long long n, Grid g, uint32_t *flag,
     return (long long)last[0] + last[1];
 }
 
-void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool far, hipStream_t s) {
+void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool far, const FarScratch &fs,
+                  hipStream_t s) {
     if (a.n <= 0 || nocc <= 0) return;
     const long long nb = (nocc + BLOCK / 64 - 1) / (BLOCK / 64);
     hipLaunchKernelGGL(k_nlist, dim3((unsigned)nb), dim3(BLOCK), 0, s, a, occ, nocc);
-    if (far) hipLaunchKernelGGL(k_nlist_far, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a);
+    if (far) {
+        hipLaunchKernelGGL(k_far_flags, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a.n, a.type, fs.flag);
+        exclusive_scan_u32(fs.flag, fs.pos, a.n, fs.scan_temp, s);
+        hipLaunchKernelGGL(k_far_compact, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a.n, fs.flag, fs.pos, fs.list, fs.count);
+        // enough waves to fill the GPU; each takes list entries in turn
+        const unsigned grid = (unsigned)std::min<long long>(nblk(a.n), 4096);
+        hipLaunchKernelGGL(k_nlist_far, dim3(grid), dim3(BLOCK), 0, s, a, fs.list, fs.count);
+    }
 }
 
 void launch_bondlist(long long n, long long stride, const uint32_t *slot, const uint32_t *inv,

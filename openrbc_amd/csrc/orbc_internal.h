@@ -265,8 +265,15 @@ struct NlistArgs {
 };
 long long launch_cell_list(const double *x, long long n, Grid g, uint32_t *flag, uint32_t *pos, uint32_t *occ,
                            void *scan_temp, hipStream_t s);
-// far: some type reaches past the first ring of cells (outer shells appended by a second kernel)
-void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool far, hipStream_t s);
+// far: some type reaches past the first ring of cells (outer shells appended by a second kernel,
+// over the compacted list of those particles; FarScratch: n-sized flag, pos, list, a device count
+// and the scan's temporary)
+struct FarScratch {
+    uint32_t *flag, *pos, *list, *count;
+    void *scan_temp;
+};
+void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool far, const FarScratch &fs,
+                  hipStream_t s);
 void set_list_cutoffs(const float *cut2_list, const int *rings, hipStream_t s);
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g,
                  unsigned *disp_max, hipStream_t s);
