@@ -115,24 +115,22 @@ __global__ __launch_bounds__(BLOCK) void k_keys(const double *x, long long n, Gr
     vals[i] = (uint32_t)i;
 }
 
+// AoS triples: one thread per component, so three consecutive threads read
+// one particle's triple and every store instruction writes a contiguous run
+__global__ __launch_bounds__(BLOCK) void k_permute3(PermuteArgs a) {
+    const long long e = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= 3 * a.n) return;
+    const long long i = e / 3;
+    const long long src = 3 * (long long)a.perm[i] + (e - 3 * i);
+    for (int k = 0; k < a.n3; ++k) a.dst3[k][e] = a.src3[k][src];
+    for (int k = 0; k < a.nf3; ++k) a.dstf3[k][e] = a.srcf3[k][src];
+}
+
+// per-particle fields
 __global__ __launch_bounds__(BLOCK) void k_permute(PermuteArgs a) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= a.n) return;
     const long long p = a.perm[i];
-    for (int k = 0; k < a.n3; ++k) {
-        const double *s = a.src3[k];
-        double *d = a.dst3[k];
-        d[3 * i] = s[3 * p];
-        d[3 * i + 1] = s[3 * p + 1];
-        d[3 * i + 2] = s[3 * p + 2];
-    }
-    for (int k = 0; k < a.nf3; ++k) {
-        const float *s = a.srcf3[k];
-        float *d = a.dstf3[k];
-        d[3 * i] = s[3 * p];
-        d[3 * i + 1] = s[3 * p + 1];
-        d[3 * i + 2] = s[3 * p + 2];
-    }
     a.type_dst[i] = a.type_src[p];
     a.id_dst[i] = a.id_src[p];
     a.slot_dst[i] = a.slot_src[p];
@@ -754,6 +752,7 @@ void launch_keys(const double *x, long long n, Grid g, uint32_t *keys, uint32_t 
 
 void launch_permute(const PermuteArgs &a, hipStream_t s) {
     if (a.n <= 0) return;
+    if (a.n3 > 0 || a.nf3 > 0) hipLaunchKernelGGL(k_permute3, dim3(nblk(3 * a.n)), dim3(BLOCK), 0, s, a);
     hipLaunchKernelGGL(k_permute, dim3(nblk(a.n)), dim3(BLOCK), 0, s, a);
 }
 
