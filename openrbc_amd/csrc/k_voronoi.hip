@@ -293,8 +293,12 @@ __global__ __launch_bounds__(BLOCK) void k_within(VoroArgs a, const float *q, lo
 // only the table is flushed with global atomics.  A key that finds the table
 // full goes straight to the global atomics.  Integer sums are exact, so the
 // grouping does not change the result.
-constexpr int SUM_CHUNK = 2048;      // particles per block
-constexpr int SUM_SLOTS = 512;       // LDS hash slots (power of two)
+#ifndef ORBC_SUM_LOG_SLOTS
+#define ORBC_SUM_LOG_SLOTS 9
+#endif
+constexpr int SUM_LOG_SLOTS = ORBC_SUM_LOG_SLOTS;
+constexpr int SUM_SLOTS = 1 << SUM_LOG_SLOTS;   // LDS hash slots
+constexpr int SUM_CHUNK = 4 * SUM_SLOTS;        // particles per block
 
 // inclusive segmented scan over runs of equal keys; true on the last lane of a run
 __device__ inline bool wave_run_reduce4(int key, unsigned long long v[4]) {
@@ -343,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_sums(VoroArgs a, unsigned long l
         }
         const bool tail = wave_run_reduce4(k, v);
         if (tail && k >= 0) {
-            unsigned h = ((unsigned)k * 2654435761u) >> (32 - 9);     // 9 = log2(SUM_SLOTS)
+            unsigned h = ((unsigned)k * 2654435761u) >> (32 - SUM_LOG_SLOTS);
             bool placed = false;
             for (int probe = 0; probe < SUM_SLOTS; ++probe) {
                 const int prev = atomicCAS(&tkey[h], -1, k);
