@@ -139,6 +139,34 @@ def main():
                         ptag=ptag, lbase=lbase, centroids=cen, laff=laff, paff=paff, bt=bt, bi=bit, bj=bjt,
                         frame=frame, topo=topo)
     print("frame fixture: %d bytes, topology %d bytes" % (len(frame), len(topo)))
+
+    # the reference's orbc-util (oracle/_ref/orbc-util) on a trajectory whose
+    # lipid count shrinks, as after stray cleanup: frame 0 with every field,
+    # frame 5 with the default fields, frame 9 with 5 lipids fewer
+    import subprocess
+    tool = os.path.join(ROOT, "oracle", "_ref", "orbc-util")
+    util = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        src = os.path.join(tmp, "traj.orbc")
+        R.save_frame(src, lx, lv, ln, laff, lbase, px, pv, pn, paff, ptype, ptag, 0, 31)
+        R.save_frame(src, lx * 1.01, lv, ln, laff, lbase, px, pv, pn, paff, ptype, ptag, 5, 7, append=True)
+        k = len(lx) - 5
+        R.save_frame(src, lx[:k] * 0.99, lv[:k], ln[:k], laff[:k], lbase, px, pv, pn, paff, ptype, ptag, 9, 3,
+                     append=True)
+        util["orbc"] = np.frombuffer(open(src, "rb").read(), np.uint8)
+        run = lambda *a: subprocess.run([tool, *a], cwd=tmp, check=True, capture_output=True)
+        run("convert", "traj.orbc", "traj.lammpstrj")
+        util["lammpstrj"] = np.frombuffer(open(os.path.join(tmp, "traj.lammpstrj"), "rb").read(), np.uint8)
+        run("convert", "traj.orbc", "traj.%d.namdbin")
+        for step in (0, 5, 9):
+            util[f"namdbin_{step}"] = np.frombuffer(open(os.path.join(tmp, f"traj.{step}.namdbin"), "rb").read(),
+                                                    np.uint8)
+        for sel in (("5",), ("0", "9")):
+            run("extract", "traj.orbc", "ext.orbc", *sel)
+            util["extract_" + "_".join(sel)] = np.frombuffer(open(os.path.join(tmp, "ext.orbc"), "rb").read(),
+                                                             np.uint8)
+    np.savez_compressed(os.path.join(HERE, "orbc_util.npz"), **util)
+    print("orbc-util fixture:", {k: len(v) for k, v in util.items()})
     print("wrote fixtures to", HERE)
 
 
