@@ -19,9 +19,9 @@ using namespace orbc;
 namespace {
 
 #ifndef ORBC_CENTROID_CELL
-#define ORBC_CENTROID_CELL 6.0
+#define ORBC_CENTROID_CELL 8.0
 #endif
-constexpr double kCentroidCell = ORBC_CENTROID_CELL;   // centroid-grid edge (~1.6 x centroid spacing)
+constexpr double kCentroidCell = ORBC_CENTROID_CELL;   // centroid-grid edge (~2 x centroid spacing)
 
 int ceil_log2(long long v) {
     int b = 0;
