@@ -151,9 +151,9 @@ __device__ int nearest_warm(const VoroArgs &a, float p0, float p1, float p2, int
 // distance); a wave whose box holds too many cells or centroids falls back to
 // nearest_warm per lane.
 #ifndef ORBC_NW_CAP
-#define ORBC_NW_CAP 256
+#define ORBC_NW_CAP 192
 #endif
-constexpr int NW_CAP = ORBC_NW_CAP;   // staged centroids per wave (384 measured slower: fewer waves resident)
+constexpr int NW_CAP = ORBC_NW_CAP;   // staged centroids per wave (larger caps measured slower: fewer waves resident)
 __global__ __launch_bounds__(BLOCK) void k_nearest_wave(VoroArgs a, const double *qd, long long nq, int32_t *out) {
     if (gate_closed(a.gate)) return;
     __shared__ float4 s_c[BLOCK / 64][NW_CAP];
