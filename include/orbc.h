@@ -74,14 +74,15 @@ typedef struct orbc_config {
     double eta;                 /* 0.01  */
     double dt;                  /* 0.01  */
     double rho;                 /* 1.05  (initialisers) */
-    int voronoi_cell_size;      /* 14    */
+    int voronoi_cell_size;      /* 14    lipids per Voronoi cell (> 0) */
     double stray_tolerance;     /* 2.5   */
     int freq_voronoi;           /* 2     */
     int freq_sort_ctrd;         /* 24    */
     int freq_sort_bond;         /* 120   */
     int freq_cleanup;           /* 60    */
     unsigned long long seed;    /* 0xBAD5EED */
-    double Q;                   /* Nose-Hoover mass; 0 => 0.01 * N (src/runtime_parameter.h:76) */
+    double Q;                   /* Nose-Hoover mass; 0 => 0.01 * N at the first Nose-Hoover step of the
+                                   uploaded system, then kept (src/runtime_parameter.h:76) */
     double zeta;                /* Nose-Hoover friction, 0 */
     double box_lo[3], box_hi[3];/* bounce-back box, +-1000 (src/runtime_parameter.h:111-115) */
     int periodic[3];            /* 0: bounce/open, 1: periodic over [box_lo, box_hi) (minimum image;
@@ -95,7 +96,8 @@ typedef struct orbc_config {
                                    orbc_voronoi_init; cells updated every freq_voronoi steps) */
     double volume_strength;     /* constrain_volume strength */
     int device;                 /* HIP device ordinal */
-    int block_size;             /* force/step kernel block size: 64/128/256/512; 0 = tuned default */
+    int block_size;             /* workgroup size of the force and step kernels: 0 (default) or 256, the
+                                   size they are built for; any other value is rejected */
     int block_halo;             /* fused steps (Langevin, Nose-Hoover second half): stage each particle
                                    block's neighbours in LDS (FP32 mirrors, or FP64 positions and
                                    directors) and walk 16-bit local indices
@@ -136,7 +138,10 @@ int orbc_get_config(const orbc_ctx *ctx, orbc_config *cfg);
 /* Container upload (LipidContainer/ProteContainer fields x,v,n,o; protein
  * type and tag; src/container.h:65-157).  AoS xyz, FP64.  `type` is ignored
  * for lipids (always 0, src/container.h:128-130); `id` is the reference tag
- * (lipids: base + i, src/container.h:122-126).  v, o may be NULL (zero). */
+ * (lipids: base + i, src/container.h:122-126).  v, o may be NULL (zero).
+ * A re-upload with the same count and the same tags (and protein types) in
+ * the same order updates the state in place; one that changes the count,
+ * permutes the tags or changes a type rebuilds the device store. */
 int orbc_set_state(orbc_ctx *ctx, int species, long long n, const double *x, const double *v,
                    const double *nrm, const double *o, const int *type, const unsigned *id);
 /* ProteContainer::bonds (src/container.h:30-34,138): {type, tag_i, tag_j} */

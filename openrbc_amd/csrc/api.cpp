@@ -102,7 +102,15 @@ IntegArgs integ_args(orbc_ctx *c) {
     return a;
 }
 
-double nh_Q(orbc_ctx *c) { return c->cfg.Q > 0 ? c->cfg.Q : 0.01 * (double)c->n; }  // set_Q, src/runtime_parameter.h:76
+// Nose-Hoover mass: cfg.Q, or 0.01 N fixed at the first Nose-Hoover step of
+// the current system, as the reference sets it once (`if (!parameter.Q)
+// set_Q(n)`, src/runtime_parameter.h:76); a later stray-lipid cleanup that
+// shrinks N keeps it
+double nh_Q(orbc_ctx *c) {
+    if (c->cfg.Q > 0) return c->cfg.Q;
+    if (!(c->nh_q > 0)) c->nh_q = 0.01 * (double)c->n;
+    return c->nh_q;
+}
 
 // ------------------------------------------------------------------ commit
 void alloc_store(orbc_ctx *c) {
@@ -290,6 +298,7 @@ void commit(orbc_ctx *c) {
     c->cur = 0;
     c->live = true;
     c->nh_started = false;
+    c->nh_q = 0.0;
     c->nbonds = 0; c->kbmax = 0;
     c->grp = c->grp_alt = nullptr;
     c->sig_group = nullptr;
@@ -643,6 +652,8 @@ void orbc_config_default(orbc_config *cfg) {
 static bool config_valid(const orbc_config &cfg_) {
     if (cfg_.dt <= 0 || cfg_.skin <= 0 || cfg_.kBT < 0 || cfg_.eta < 0) return false;
     if (cfg_.block_halo < 0 || cfg_.block_halo > 2) return false;
+    if (cfg_.block_size != 0 && cfg_.block_size != BLOCK) return false;   // the kernels are built for 256
+    if (cfg_.voronoi_cell_size <= 0) return false;                         // lipids per Voronoi cell
     if (cfg_.precision != ORBC_FP64 && cfg_.precision != ORBC_MIXED) return false;
     for (int d = 0; d < 3; ++d)
         if (cfg_.periodic[d] && !(cfg_.box_hi[d] > cfg_.box_lo[d])) return false;
@@ -697,7 +708,7 @@ int orbc_set_config_internal(orbc_ctx *c, const orbc_config &cfg) {
     c->cfg = cfg;
     c->group_kbt.clear();
     for (auto &S : c->sp) S.group.clear();
-    c->step = 0; c->nh_started = false; c->deleted = 0; c->last_cleanup = -1;
+    c->step = 0; c->nh_started = false; c->nh_q = 0.0; c->deleted = 0; c->last_cleanup = -1;
     c->vol_target_set = false; c->list_life = 0.0; c->since_build = 0;
     return ORBC_OK;
 }
@@ -749,7 +760,14 @@ int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const
         if (n < 0 || (n > 0 && (!x || !nrm || !id))) return fail(c, ORBC_ERR_INVALID, "x, nrm and id are required");
         if (species == ORBC_PROTEIN && n > 0 && !type) return fail(c, ORBC_ERR_INVALID, "protein type is required");
         Species &S = c->sp[species];
-        const bool same = c->live && S.n == n;
+        bool same = c->live && S.n == n;
+        if (same && n > 0) {
+            // the device keeps tags and types per slot: a re-upload that permutes
+            // them (the reference reorders its containers, src/reorder.h:73-149)
+            // re-commits the store, so bonds, noise keys and force-field rows follow
+            if (!std::equal(id, id + n, S.id.begin())) same = false;
+            if (species == ORBC_PROTEIN && !std::equal(type, type + n, S.type.begin())) same = false;
+        }
         c->vol_target_set = false;          // a volume target "< 0" is re-read from the new state
         S.n = n;
         S.x.assign(x, x + 3 * n);
@@ -779,7 +797,7 @@ int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const
             c->need_rebuild = true;
             sync(c);
         } else if (c->live) {
-            c->live = false;   // counts changed: re-commit on next use
+            c->live = false;   // counts, tags or types changed: re-commit on next use
             voronoi_free(c);
             c->mem.free_all();
         }

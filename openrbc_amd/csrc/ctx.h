@@ -116,6 +116,7 @@ struct orbc_ctx {
     bool moved = false;          // positions changed by a standalone kernel since the last check
     long long step = 0, rebuilds = 0, last_pairs = 0, since_build = 0;
     bool nh_started = false;
+    double nh_q = 0.0;           // Nose-Hoover mass resolved at its first use (cfg.Q == 0: 0.01 N then)
     unsigned types_present = 0;  // bit t set if some particle has type t
     int batch = 32;              // fused steps launched per host synchronisation
     // kernel timing (orbc_set_timing)

@@ -257,6 +257,7 @@ int orbc_save_restart(orbc_ctx *c, const char *path) {
         sync(c);
         cfg.zeta = zeta;
         if (c->vol_target_set) cfg.volume_target = c->vol_target;
+        if (!(cfg.Q > 0) && c->nh_q > 0) cfg.Q = c->nh_q;       // the Nose-Hoover mass in use
         std::FILE *fp = std::fopen(path, "wb");
         if (!fp) return fail(c, ORBC_ERR_INVALID, std::string("cannot open ") + path);
         std::fwrite(kRestartMagic, 1, 8, fp);

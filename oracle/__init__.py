@@ -19,6 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "liborbc_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libref_openrbc.so")
+REF_OPENRBC = os.path.join(HERE, "_ref", "openrbc")
 
 _d = C.POINTER(C.c_double)
 _f = C.POINTER(C.c_float)
@@ -314,6 +315,15 @@ class Reference:
                                      _i32, _i32, C.c_int, C.c_int, C.c_int]
         L.ref_save_topology.argtypes = [C.c_char_p, C.c_int, _f, C.c_int, C.c_int, _f, _i32, _i32, C.c_int,
                                         _i32, _i32, _i32]
+        L.ref_forces_cells.restype = C.c_int
+        L.ref_forces_cells.argtypes = [C.c_int, _f, _f, C.c_int, _f, _f, _i32, _i32, C.c_int, _i32, _i32, _i32,
+                                       C.c_int, _f, _i32, _i32, _f, _f, _f, _f]
+        L.ref_init_rbc.restype = C.c_double
+        L.ref_init_rbc.argtypes = [C.c_char_p, C.c_int, C.c_double, C.POINTER(C.c_long), _f, _f, _f, _f, _i32,
+                                   _i32, _i32, _i32, _i32]
+        L.ref_langevin_lipids.restype = None
+        L.ref_langevin_lipids.argtypes = [C.c_int, _f, _f, _f, C.c_double, C.c_double, C.c_double, C.c_int,
+                                          C.c_int, C.c_int, _f, _f, _f]
         L.ref_run.restype = C.c_double
         L.ref_run.argtypes = [C.c_int, _f, _f, C.c_int, _f, _f, _i32, _i32, C.c_int, _i32, _i32, _i32,
                               C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -346,6 +356,46 @@ class Reference:
                           len(bt), _p(bt, _i32), _p(bi, _i32), _p(bj, _i32), int(do_pair), int(do_bond),
                           _p(lf, _f), _p(lt, _f), _p(pf, _f), _p(pt, _f))
         return lf, lt, pf, pt
+
+    def forces_cells(self, lx, ln, px, pn, ptype, ptag, bonds_tag, centroids, lcell_start, pcell_start):
+        """compute_pairwise_fused over Voronoi cells (containers sorted by cell,
+        cell_start offsets) + compute_bonded.  Returns lf, lt, pf, pt (FP32)."""
+        lx = _c(lx, np.float32); ln = _c(ln, np.float32); px = _c(px, np.float32); pn = _c(pn, np.float32)
+        pty = _c(ptype, np.int32); ptg = _c(ptag, np.int32)
+        bt, bi, bj = (_c(b, np.int32) for b in bonds_tag)
+        c = _c(centroids, np.float32); ls = _c(lcell_start, np.int32); ps = _c(pcell_start, np.int32)
+        nl, np_ = len(lx), len(px)
+        lf = np.zeros((nl, 3), np.float32); lt = np.zeros((nl, 3), np.float32)
+        pf = np.zeros((np_, 3), np.float32); pt = np.zeros((np_, 3), np.float32)
+        self.L.ref_forces_cells(nl, _p(lx, _f), _p(ln, _f), np_, _p(px, _f), _p(pn, _f), _p(pty, _i32),
+                                _p(ptg, _i32), len(bt), _p(bt, _i32), _p(bi, _i32), _p(bj, _i32), len(c), _p(c, _f),
+                                _p(ls, _i32), _p(ps, _i32), _p(lf, _f), _p(lt, _f), _p(pf, _f), _p(pt, _f))
+        return lf, lt, pf, pt
+
+    def init_rbc(self, prefix, seed, rho=1.05):
+        """init_rbc on the mesh files prefix.{vert,bond,face}.txt (FP32 outputs)."""
+        cnt = (C.c_long * 3)()
+        self.L.ref_init_rbc(prefix.encode(), int(seed), float(rho), cnt, *([None] * 9))
+        nl, np_, nb = cnt[0], cnt[1], cnt[2]
+        out = dict(lx=np.zeros((nl, 3), np.float32), ln=np.zeros((nl, 3), np.float32),
+                   px=np.zeros((np_, 3), np.float32), pn=np.zeros((np_, 3), np.float32),
+                   ptype=np.zeros(np_, np.int32), ptag=np.zeros(np_, np.int32), bt=np.zeros(nb, np.int32),
+                   bi=np.zeros(nb, np.int32), bj=np.zeros(nb, np.int32))
+        area = self.L.ref_init_rbc(prefix.encode(), int(seed), float(rho), cnt,
+                                   *[_p(out[k], _f) for k in ("lx", "ln", "px", "pn")],
+                                   *[_p(out[k], _i32) for k in ("ptype", "ptag", "bt", "bi", "bj")])
+        out["area"] = area
+        return out
+
+    def langevin_lipids(self, lx, ln, lv, kT, eta, dt, seed, nsteps, every):
+        """verlet_langevin run of a lipid-only system with exact pairs; snapshots
+        (x, n, v) every `every` steps, shape (nsteps // every, n, 3) each."""
+        lx = _c(lx, np.float32); ln = _c(ln, np.float32); lv = _c(lv, np.float32)
+        m = nsteps // every
+        xs = np.zeros((m, len(lx), 3), np.float32); ns = np.zeros_like(xs); vs = np.zeros_like(xs)
+        self.L.ref_langevin_lipids(len(lx), _p(lx, _f), _p(ln, _f), _p(lv, _f), kT, eta, dt, int(seed),
+                                   int(nsteps), int(every), _p(xs, _f), _p(ns, _f), _p(vs, _f))
+        return xs, ns, vs
 
     def _state_args(self, L, P, ptype):
         return ([len(L["x"])] + [_p(L[k], _f) for k in ("x", "v", "n", "o", "f", "t")] + [len(P["x"])] +

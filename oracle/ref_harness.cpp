@@ -24,6 +24,7 @@
 #include "assign_temperature.h"
 #include "trajectory.h"
 #include "topology.h"
+#include "init_rbc.h"
 
 #include <cstdio>
 #include <fstream>
@@ -354,6 +355,120 @@ void ref_save_topology(const char *path, int nl, const float *lx, int lbase, int
     protein.bonds.resize(0);
     for (int k = 0; k < nb; ++k) protein.bonds.emplace_back(bt[k], bi[k], bj[k]);
     save_topology(std::ofstream(path), *param, protein, lipid);
+    delete param;
+}
+
+// compute_pairwise_fused (src/compute_pairwise_fused.h:238-320) over the given
+// Voronoi cells: the containers arrive sorted by cell (lcell_start /
+// pcell_start: n_cells + 1 offsets each), the centroids fix the stencils
+// (radii 9 / 8 / 6 for protein-protein / protein-lipid / lipid-lipid), then
+// compute_bonded.  One thread: every stencil cell is inside the thread's
+// range (newton_on).  Outputs accumulate from zero, in the input order.
+int ref_forces_cells(int nl, const float *lx, const float *ln, int np, const float *px, const float *pn,
+                     const int *ptype, const int *ptag, int nb, const int *bt, const int *bi_tag,
+                     const int *bj_tag, int ncell, const float *centroids, const int *lcell_start,
+                     const int *pcell_start, float *lf, float *lt, float *pf, float *pt) {
+    omp_set_num_threads(1);
+    LipidContainer lipid("lipid");
+    ProteContainer protein("protein");
+    load(lipid, nl, lx, nullptr, ln, nullptr, nullptr, nullptr);
+    load(protein, np, px, nullptr, pn, nullptr, nullptr, nullptr);
+    load_protein_meta(protein, np, ptype, ptag);
+    lipid.tag.set_base(np + 1);
+    protein.bonds.resize(0);
+    for (int k = 0; k < nb; ++k) protein.bonds.emplace_back(bt[k], bi_tag[k], bj_tag[k]);
+    protein.tag2idx.build_map(protein);
+
+    VoronoiDiagram voronoi(ncell);
+    voronoi.centroids.resize(ncell);
+    for (int k = 0; k < ncell; ++k)
+        voronoi.centroids[k] = vector<real, 3>(centroids[3 * k], centroids[3 * k + 1], centroids[3 * k + 2]);
+    voronoi.tree.build(voronoi.centroids);
+    VCellList cl, cp;
+    cl.n_cells = cp.n_cells = ncell;
+    cl.cell_start.resize(ncell + 1);
+    cp.cell_start.resize(ncell + 1);
+    for (int k = 0; k <= ncell; ++k) { cl.cell_start[k] = lcell_start[k]; cp.cell_start[k] = pcell_start[k]; }
+
+    compute_pairwise_fused(voronoi, lipid, protein, cl, cp);
+    if (nb > 0) {
+        Service<BalancerMap>::call()["protein-bonds"].set_range(nb);
+        compute_bonded(protein);
+    }
+    store(lipid, nullptr, nullptr, nullptr, nullptr, lf, lt);
+    store(protein, nullptr, nullptr, nullptr, nullptr, pf, pt);
+    return 0;
+}
+
+// init_rbc (src/init_rbc.h:30-337) on the mesh files prefix.{vert,bond,face}.txt
+// with --random-seed `seed`.  Called with null arrays it only returns the
+// counts (nl, np, nb); the second call fills the arrays (lipids in generation
+// order, face by face; proteins in tag order; bonds as generated).  Returns
+// the total mesh area.
+double ref_init_rbc(const char *prefix, int seed, double rho, long *counts, float *lx, float *ln, float *px,
+                    float *pn, int *ptype, int *ptag, int *bt, int *bi, int *bj) {
+    omp_set_num_threads(1);
+    RTParameter *param = make_param(0.22, 0.01, 0.01, seed);
+    param->init = "trimesh";
+    param->mesh = prefix;
+    param->rho = (float)rho;
+    LipidContainer lipid("lipid");
+    ProteContainer protein("protein");
+    double area;
+    {
+        QuietCout q;
+        area = init_rbc(lipid, protein, *param, 0);    // its progress lines go to stdout
+        std::fflush(stdout);
+    }
+    counts[0] = (long)lipid.size();
+    counts[1] = (long)protein.size();
+    counts[2] = (long)protein.bonds.size();
+    if (lx) {
+        store(lipid, lx, nullptr, ln, nullptr, nullptr, nullptr);
+        store(protein, px, nullptr, pn, nullptr, nullptr, nullptr);
+        for (std::size_t i = 0; i < protein.size(); ++i) { ptype[i] = protein.type[i]; ptag[i] = protein.tag[i]; }
+        for (std::size_t k = 0; k < protein.bonds.size(); ++k) {
+            bt[k] = protein.bonds[k].type; bi[k] = protein.bonds[k].i; bj[k] = protein.bonds[k].j;
+        }
+    }
+    delete param;
+    return area;
+}
+
+// Langevin statistics of the reference: nsteps of compute_pairwise_fused over
+// one Voronoi cell holding every lipid (the exact pair set) and
+// integrate(verlet_langevin) (src/integrate_langevin.h:99-149) from the given
+// lipid state, on one thread.  Every `every` steps the state is copied out:
+// xs, ns, vs hold (nsteps / every) snapshots of 3 nl floats each.
+void ref_langevin_lipids(int nl, const float *lx, const float *ln, const float *lv, double kT, double eta, double dt,
+                         int seed, int nsteps, int every, float *xs, float *ns, float *vs) {
+    omp_set_num_threads(1);
+    RTParameter *param = make_param(kT, eta, dt, seed);
+    LipidContainer lipid("lipid");
+    ProteContainer protein("protein");
+    load(lipid, nl, lx, lv, ln, nullptr, nullptr, nullptr);
+    protein.resize(0);
+    VoronoiDiagram voronoi(1);
+    voronoi.centroids.resize(1);
+    voronoi.centroids[0] = vector<real, 3>(0, 0, 0);
+    voronoi.tree.build(voronoi.centroids);
+    VCellList cl, cp;
+    cl.n_cells = cp.n_cells = 1;
+    cl.cell_start.resize(2);
+    cp.cell_start.resize(2);
+    cl.cell_start[0] = 0; cl.cell_start[1] = nl;
+    cp.cell_start[0] = 0; cp.cell_start[1] = 0;
+    integrate(clear_force(), lipid, protein);
+    long k = 0;
+    for (int step = 0; step < nsteps; ++step) {
+        param->nstep = step;
+        compute_pairwise_fused(voronoi, lipid, protein, cl, cp);
+        integrate(verlet_langevin(*param), lipid, protein);
+        if ((step + 1) % every == 0) {
+            store(lipid, xs + 3L * nl * k, vs + 3L * nl * k, ns + 3L * nl * k, nullptr, nullptr, nullptr);
+            ++k;
+        }
+    }
     delete param;
 }
 

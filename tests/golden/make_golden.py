@@ -20,8 +20,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-from oracle import Reference  # noqa: E402
-from _systems import patch, split  # noqa: E402
+from oracle import REF_OPENRBC, Oracle, Reference  # noqa: E402
+from _systems import contact_patch, patch, split  # noqa: E402
 
 
 def containers(s):
@@ -33,6 +33,13 @@ def containers(s):
 
 def main():
     R = Reference()
+    if len(sys.argv) > 1:          # only the named fixture groups
+        for name in sys.argv[1:]:
+            if name == "minimize_fixture":
+                minimize_fixture()
+            else:
+                globals()[name](R)
+        return
     out = {}
     ff = R.ff_tables()
     np.savez_compressed(os.path.join(HERE, "ff_tables.npz"), **ff)
@@ -167,7 +174,173 @@ def main():
                                                              np.uint8)
     np.savez_compressed(os.path.join(HERE, "orbc_util.npz"), **util)
     print("orbc-util fixture:", {k: len(v) for k, v in util.items()})
+    contact_fixtures(R)
+    voronoi_stencil_fixture(R)
+    init_rbc_fixture(R)
+    minimize_fixture()
+    langevin_stats_fixture(R)
     print("wrote fixtures to", HERE)
+
+
+def contact_fixtures(R):
+    """compute_pairwise_fused + compute_bonded on the contact patch, whose
+    pairs include every WCA and protein-protein kind (one cell: all pairs)."""
+    s = contact_patch(seed=0)
+    lip, prot, bonds = containers(s)
+    lf, lt, pf, pt = R.forces(s["x"][lip], s["n"][lip], s["x"][prot], s["n"][prot], s["type"][prot],
+                              s["id"][prot], bonds)
+    np.savez_compressed(os.path.join(HERE, "forces_contacts.npz"),
+                        lx=s["x"][lip].astype(np.float32), ln=s["n"][lip].astype(np.float32),
+                        px=s["x"][prot].astype(np.float32), pn=s["n"][prot].astype(np.float32),
+                        ptype=s["type"][prot].astype(np.int32), ptag=s["id"][prot].astype(np.int32),
+                        bt=bonds[0], bi=bonds[1], bj=bonds[2], lf=lf, lt=lt, pf=pf, pt=pt)
+    print("contact patch: %d lipids, %d proteins" % (len(lip), len(prot)))
+
+
+def voronoi_stencil_fixture(R):
+    """compute_pairwise_fused over 20+ Voronoi cells with given centroids (the
+    reference's cell-stencil sweep, radii 9 / 8 / 6): cells of ~30 lipids, so
+    that every radius drops pairs that lie within the cutoff."""
+    s = contact_patch(seed=3)
+    x = s["x"].astype(np.float32)
+    t = s["type"]
+    rng = np.random.default_rng(5)
+    lipids = np.where(t == 0)[0]
+    cen = x[rng.choice(lipids, len(lipids) // 30, replace=False)].copy()
+    aff = R.partition(x, cen)
+    lip, prot = split(s)
+    lo = lip[np.argsort(aff[lip], kind="stable")]
+    po = prot[np.argsort(aff[prot], kind="stable")]
+    ncell = len(cen)
+    ls = np.searchsorted(aff[lo], np.arange(ncell + 1)).astype(np.int32)
+    ps = np.searchsorted(aff[po], np.arange(ncell + 1)).astype(np.int32)
+    bt, bi, bj = s["bonds"]
+    tag = s["id"].astype(np.int32)
+    bonds = (bt.astype(np.int32), tag[bi], tag[bj])
+    lf, lt, pf, pt = R.forces_cells(x[lo], s["n"][lo], x[po], s["n"][po], t[po], s["id"][po], bonds, cen, ls, ps)
+    np.savez_compressed(os.path.join(HERE, "voronoi_forces.npz"),
+                        lx=x[lo], ln=s["n"][lo].astype(np.float32), px=x[po], pn=s["n"][po].astype(np.float32),
+                        ptype=t[po].astype(np.int32), ptag=s["id"][po].astype(np.int32), bt=bonds[0], bi=bonds[1],
+                        bj=bonds[2], centroids=cen, laff=aff[lo].astype(np.int32), paff=aff[po].astype(np.int32),
+                        lcell_start=ls, pcell_start=ps, lf=lf, lt=lt, pf=pf, pt=pt)
+    print("voronoi stencil: %d cells" % ncell)
+
+
+def _write_mesh(prefix, v, e, f):
+    np.savetxt(prefix + ".vert.txt", v, fmt="%.17g")
+    np.savetxt(prefix + ".bond.txt", e, fmt="%d")
+    np.savetxt(prefix + ".face.txt", f, fmt="%d")
+
+
+def init_rbc_fixture(R):
+    """init_rbc on the icosphere of subdivision 1 (42 vertices), seed 1."""
+    import tempfile
+    import openrbc_amd.systems as systems
+    v, e, f = systems.icosphere(1)
+    with tempfile.TemporaryDirectory() as d:
+        prefix = os.path.join(d, "ico1")
+        _write_mesh(prefix, v, e, f)
+        g = R.init_rbc(prefix, 1, 1.05)
+    np.savez_compressed(os.path.join(HERE, "init_rbc_ico1.npz"), vert=v, edge=e.astype(np.int32),
+                        face=f.astype(np.int32), seed=1, rho=1.05, area=g["area"],
+                        **{k: g[k] for k in ("lx", "ln", "px", "pn", "ptype", "ptag", "bt", "bi", "bj")})
+    print("init_rbc: %d lipids, %d proteins, %d bonds" % (len(g["lx"]), len(g["px"]), len(g["bt"])))
+
+
+def _run_openrbc(v, e, f, extra):
+    """The reference program on a mesh: init, Voronoi init, one minimisation
+    step, frames before and after it with every field."""
+    import subprocess
+    import tempfile
+    import openrbc_amd as orbc
+    with tempfile.TemporaryDirectory() as d:
+        _write_mesh(os.path.join(d, "mesh"), v, e, f)
+        cmd = [REF_OPENRBC, "-i", "trimesh", "-m", "mesh", "-E", "1", "-t", "0", "-d", "1", "--dump-field", "31",
+               "--random-seed", "1"] + extra
+        subprocess.run(cmd, cwd=d, check=True, capture_output=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        return [orbc.read_frame(os.path.join(d, "cell.orbc"), k) for k in range(2)]
+
+
+def _match_frames(a, b):
+    """Particles of frame a in frame b: proteins by tag; lipids, whose tags
+    are positional in the reference, by nearest position (they move less than
+    the minimiser's 0.05 cap)."""
+    from scipy.spatial import cKDTree
+    nl = int((a["type"] == 0).sum())
+    ia = np.argsort(a["tag"][nl:]) + nl
+    ib = np.argsort(b["tag"][nl:]) + nl
+    dd, jj = cKDTree(b["x"][:nl]).query(a["x"][:nl])
+    assert len(np.unique(jj)) == nl and dd.max() < 0.2
+    return np.concatenate([np.arange(nl), ia]), np.concatenate([jj, ib])
+
+
+def minimize_fixture():
+    """The capped steepest-descent step of the reference's minimisation loop
+    (src/openrbc.cpp:88-146), run by the reference program itself on the
+    icosphere of subdivision 0: frame 0 (x0, n0) and frame 1 (x1, n1 and the
+    force f0 that moved it).  'capped' uses the default caps (0.05 / 0.05),
+    'free' caps of 1e9 (every step is dt; its lipids move too far to be
+    matched by position, so only proteins are kept)."""
+    import openrbc_amd.systems as systems
+    v, e, f = systems.icosphere(0)
+    out = dict(vert=v, edge=e.astype(np.int32), face=f.astype(np.int32))
+    for name, extra in (("capped", []), ("free", ["--opt-max-move", "1e9", "--opt-max-rotate", "1e9"])):
+        a, b = _run_openrbc(v, e, f, extra)
+        if name == "capped":
+            A, B = _match_frames(a, b)
+        else:
+            nl = int((a["type"] == 0).sum())
+            A, B = np.argsort(a["tag"][nl:]) + nl, np.argsort(b["tag"][nl:]) + nl
+        out.update({f"{name}_x0": a["x"][A], f"{name}_n0": a["n"][A], f"{name}_x1": b["x"][B],
+                    f"{name}_n1": b["n"][B], f"{name}_f0": b["f"][B], f"{name}_type": a["type"][A].astype(np.int32),
+                    f"{name}_tag": np.where(a["type"][A] == 0, -1, a["tag"][A]).astype(np.int32)})
+    np.savez_compressed(os.path.join(HERE, "minimize.npz"), **out)
+    print("minimize: %d particles (capped), %d proteins (free)" % (len(out["capped_x0"]), len(out["free_x0"])))
+
+
+def vesicle_observables(O, xs, ns, vs):
+    """Per snapshot of a lipid vesicle: temperature sum m v^2 / 3N (m = 1),
+    potential energy per particle (closed-form pair energies) and area per
+    lipid (tests/_stats.py)."""
+    from _stats import area_per_lipid
+    T, E, A = [], [], []
+    for x, n, v in zip(xs, ns, vs):
+        x = x.astype(np.float64); n = n.astype(np.float64); v = v.astype(np.float64)
+        N = len(x)
+        T.append(np.sum(v * v) / (3 * N))
+        _, _, e, _ = O.forces(x, n, np.zeros(N, np.int32))
+        E.append(e.sum() / N)
+        A.append(area_per_lipid(x))
+    return np.array(T), np.array(E), np.array(A)
+
+
+def langevin_stats_fixture(R):
+    """Reference Langevin statistics: a lipid vesicle of 1900 lipids (radius
+    12, minimised by the oracle's capped descent), Maxwell velocities at
+    kT = 0.22, then 30000 steps of the reference's verlet_langevin with exact
+    pairs; T, E/N and area per lipid every 20 steps (the first 6000 steps are
+    relaxation)."""
+    import openrbc_amd.systems as systems
+    O = Oracle()
+    sp = systems.random_sphere(r=12.0, seed=7)
+    x, n = sp["lipid_x"].copy(), sp["lipid_n"].copy()
+    N = len(x)
+    ty = np.zeros(N, np.int32)
+    for _ in range(400):
+        f, t, _, _ = O.forces(x, n, ty)
+        st = dict(x=x, n=n, f=f, t=np.cross(n, t), type=ty)
+        O.minimize_move(st, 0.05, 0.05, 0.01)
+        x, n = st["x"], st["n"]
+    rng = np.random.default_rng(3)
+    v = rng.normal(0, np.sqrt(0.22), (N, 3))
+    x32, n32, v32 = x.astype(np.float32), n.astype(np.float32), v.astype(np.float32)
+    every, nsteps = 20, 30000
+    xs, ns, vs = R.langevin_lipids(x32, n32, v32, 0.22, 0.01, 0.01, 11, nsteps, every)
+    T, E, A = vesicle_observables(O, xs, ns, vs)
+    np.savez_compressed(os.path.join(HERE, "langevin_stats.npz"), x=x32, n=n32, v=v32, kT=0.22, eta=0.01, dt=0.01,
+                        every=every, nsteps=nsteps, relax=6000, T=T, E=E, A=A)
+    k = 6000 // every
+    print("langevin stats: <T> %.5f <E/N> %.5f <A/N> %.5f" % (T[k:].mean(), E[k:].mean(), A[k:].mean()))
 
 
 if __name__ == "__main__":

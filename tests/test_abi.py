@@ -47,6 +47,13 @@ def test_invalid_configuration_is_an_error_not_an_exit():
     assert L.orbc_create(C.byref(cfg), C.byref(h)) == 1        # ORBC_ERR_INVALID
     cfg = orbc.default_config(dt=-1.0)
     assert L.orbc_create(C.byref(cfg), C.byref(h)) == 1
+    # lipids per Voronoi cell must be positive (it divides the lipid count)
+    for bad in (0, -3):
+        cfg = orbc.default_config(voronoi_cell_size=bad)
+        assert L.orbc_create(C.byref(cfg), C.byref(h)) == 1
+    # the kernels are built for workgroups of 256
+    cfg = orbc.default_config(block_size=128)
+    assert L.orbc_create(C.byref(cfg), C.byref(h)) == 1
 
 
 def test_create_without_gpu_fails_cleanly():

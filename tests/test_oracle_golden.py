@@ -133,3 +133,130 @@ def test_voronoi_queries_match_reference(oracle):
         d = (cen - q[k]).astype(np.float32) ** 2
         got = np.where((d[:, 0] + d[:, 1]) + d[:, 2] < np.float32(81.0))[0]
         assert np.array_equal(np.sort(exp), got)
+
+
+# ---- WCA and protein-protein pairs (the contact patch) ------------------------------------
+def _kinds_of(oracle, x, t, pairs):
+    """Pair kind of each interacting pair: (min type, max type, 'poly' | 'lj')."""
+    ff = oracle.ff()
+    a, b = pairs[:, 0], pairs[:, 1]
+    r2 = np.sum((x[a] - x[b]) ** 2, 1)
+    poly = r2 < ff["cut2_poly"][t[a], t[b]]
+    return [(int(min(p, q)), int(max(p, q)), "poly" if pl else "lj") for p, q, pl in zip(t[a], t[b], poly)]
+
+
+def test_contact_forces_match_reference(oracle):
+    """compute_pairwise_fused + compute_bonded of the reference on a patch that
+    holds every WCA pair (lipid-actin, lipid-spectrin, band-3-actin,
+    band-3-spectrin) and every protein-protein polynomial pair among types
+    1-3: the oracle reproduces it to FP32 accuracy, and it would not without
+    its WCA or protein-protein terms (src/pairwise_kernel_fused.h:63-97)."""
+    z = load("forces_contacts.npz")
+    x, n, t, bonds, nl = combined(z)
+    pairs = oracle.pairs_cell(x, t)
+    kinds = _kinds_of(oracle, x, t, pairs)
+    need = [(0, 4, "lj"), (0, 5, "lj"), (1, 4, "lj"), (1, 5, "lj"), (2, 4, "lj"), (2, 5, "lj"),
+            (1, 1, "poly"), (1, 2, "poly"), (1, 3, "poly"), (2, 2, "poly"), (2, 3, "poly"), (3, 3, "poly"),
+            (0, 1, "poly"), (0, 2, "poly"), (0, 3, "poly")]
+    for k in need:
+        assert kinds.count(k) > 0, k
+    f, tq, _, _ = oracle.forces(x, n, t, pairs=pairs, bonds=bonds)
+    F = np.vstack([z["lf"], z["pf"]]); T = np.vstack([z["lt"], z["pt"]])
+    scale_f = np.sqrt(np.mean(np.sum(f * f, 1)))
+    scale_t = np.sqrt(np.mean(np.sum(tq * tq, 1)))
+    assert np.abs(f - F).max() / scale_f < 2e-5
+    assert np.abs(tq - T).max() / scale_t < 2e-5
+    # the comparison is sensitive to each of the two terms
+    for drop in ("lj", "pp"):
+        keep = np.array([not ((drop == "lj" and k[2] == "lj") or (drop == "pp" and k[2] == "poly" and k[0] > 0))
+                         for k in kinds])
+        g, _, _, _ = oracle.forces(x, n, t, pairs=pairs[keep], bonds=bonds)
+        assert np.abs(g - F).max() / scale_f > 100 * 2e-5, drop
+
+
+# ---- the Voronoi cell stencil over many cells ----------------------------------------------
+def test_voronoi_stencil_forces_match_reference(oracle):
+    """The reference's cell-stencil sweep over 22 Voronoi cells with given
+    centroids (src/compute_pairwise_fused.h:253-316): the oracle's stencil
+    pair set (orc_pairs_voronoi_fp64: centroid distance < 9 / 8 / 6 for
+    protein-protein / protein-lipid / lipid-lipid) gives the reference's
+    forces; the cells are large enough that every radius drops pairs within
+    the cutoff, and restoring the dropped pairs of any one kind breaks the
+    agreement (so a swapped radius would be caught)."""
+    z = load("voronoi_forces.npz")
+    x, n, t, bonds, nl = combined(z)
+    aff = np.concatenate([z["laff"], z["paff"]])
+    cen = z["centroids"]
+    vor = oracle.pairs_voronoi(x, t, aff, cen)
+    full = oracle.pairs_cell(x, t)
+    f, tq, _, _ = oracle.forces(x, n, t, pairs=vor, bonds=bonds)
+    F = np.vstack([z["lf"], z["pf"]]); T = np.vstack([z["lt"], z["pt"]])
+    sf = np.sqrt(np.mean(np.sum(f * f, 1))); st = np.sqrt(np.mean(np.sum(tq * tq, 1)))
+    assert np.abs(f - F).max() / sf < 2e-5
+    assert np.abs(tq - T).max() / st < 2e-5
+    vs = set(map(tuple, np.sort(vor, 1).tolist()))
+    dropped = np.array([p for p in map(tuple, np.sort(full, 1).tolist()) if p not in vs], np.int64)
+    kind = np.where((t[dropped[:, 0]] == 0) & (t[dropped[:, 1]] == 0), 0,
+                    np.where((t[dropped[:, 0]] == 0) | (t[dropped[:, 1]] == 0), 1, 2))
+    for k in (0, 1, 2):                     # lipid-lipid, protein-lipid, protein-protein
+        assert (kind == k).sum() > 0, k
+        g, _, _, _ = oracle.forces(x, n, t, pairs=np.vstack([vor, dropped[kind == k]]), bonds=bonds)
+        assert np.abs(g - F).max() / sf > 50 * 2e-5, k
+
+
+# ---- the minimiser (src/openrbc.cpp:110-132), from the reference program's own frames -------
+def test_minimize_move_free_matches_reference(oracle):
+    """Caps of 1e9: every particle moves by x += f dt / m with the default dt
+    (proteins of all four kinds and masses); spectrin directors, which the
+    initialiser sets to un-normalised positions, come out normalised."""
+    z = load("minimize.npz")
+    x0, n0 = z["free_x0"].astype(np.float64), z["free_n0"].astype(np.float64)
+    f0, ty = z["free_f0"].astype(np.float64), z["free_type"]
+    st = dict(x=x0.copy(), n=n0.copy(), f=f0, t=np.zeros_like(f0), type=ty.astype(np.int32))
+    oracle.minimize_move(st, 1e9, 1e9, 0.01)
+    assert set(np.unique(ty).tolist()) == {2, 3, 4, 5}
+    assert np.abs(st["x"] - z["free_x1"]).max() < 1e-5 * max(1.0, np.abs(x0).max()) / 100
+    sp = ty == 5
+    assert np.abs(st["n"][sp] - z["free_n1"][sp]).max() < 1e-6
+    # the step is not trivially zero
+    assert np.abs(z["free_x1"] - z["free_x0"]).max() > 0.05
+
+
+def test_minimize_move_capped_matches_reference(oracle):
+    """Default caps (0.05 / 0.05): every particle's step is capped either by
+    its displacement or by its director rotation, as the reference's rule
+    min(dr_opt / |f/m|, dn_opt / |n x t x n|) makes it.  The oracle's move,
+    given the reference's force and the director force of its own exact pair
+    set (for the particles whose force the reference's stencil reproduces),
+    gives the reference's positions and directors."""
+    z = load("minimize.npz")
+    x0, n0 = z["capped_x0"].astype(np.float64), z["capped_n0"].astype(np.float64)
+    x1, n1 = z["capped_x1"].astype(np.float64), z["capped_n1"].astype(np.float64)
+    f0, ty, tag = z["capped_f0"].astype(np.float64), z["capped_type"], z["capped_tag"]
+    m = oracle.ff()["mass"][ty]
+    unit = np.abs(np.linalg.norm(n0, axis=1) - 1) < 1e-5
+    dx = np.linalg.norm(x1 - x0, axis=1)
+    w = n1 / np.sum(n1 * n0, 1)[:, None] - n0          # director update before normalisation
+    dw = np.linalg.norm(w, axis=1)
+    tcap = np.abs(dx / 0.05 - 1) < 1e-4
+    rcap = np.abs(dw / 0.05 - 1) < 1e-4
+    assert (tcap | rcap)[unit].all() and tcap.sum() > 0 and rcap.sum() > 0
+    assert (dw[tcap & unit] <= 0.05 * (1 + 1e-4)).all() and (dx[rcap] <= 0.05 * (1 + 1e-4)).all()
+    # director force from the oracle: bonds of the reference initialiser by tag
+    import openrbc_amd.systems as systems
+    import openrbc_amd as orbc
+    s = orbc.init_rbc(z["vert"], z["edge"], z["face"], rho=1.05, seed=1)
+    tag2i = {int(q): i for i, q in enumerate(tag) if q >= 0}
+    bi = np.array([tag2i[int(q)] for q in s["bond_i"]]); bj = np.array([tag2i[int(q)] for q in s["bond_j"]])
+    fo, to, _, _ = oracle.forces(x0, n0, ty.astype(np.int32), bonds=(s["bond_type"], bi, bj))
+    good = unit & (np.linalg.norm(fo - f0, axis=1) / np.maximum(np.linalg.norm(f0, axis=1), 1.0) < 1e-4)
+    assert good.sum() > 0.8 * len(ty)
+    st = dict(x=x0.copy(), n=n0.copy(), f=f0, t=np.cross(n0, to), type=ty.astype(np.int32))
+    oracle.minimize_move(st, 0.05, 0.05, 0.01)
+    assert np.abs(st["x"] - x1)[good].max() < 1e-5
+    assert np.abs(st["n"] - n1)[good].max() < 2e-6
+    # a reversed director update would move every director the other way
+    st2 = dict(x=x0.copy(), n=n0.copy(), f=f0, t=-np.cross(n0, to), type=ty.astype(np.int32))
+    oracle.minimize_move(st2, 0.05, 0.05, 0.01)
+    assert np.median(np.linalg.norm(st2["n"] - n1, axis=1)[good]) > 100 * 2e-6
+    _ = systems
