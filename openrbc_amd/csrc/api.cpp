@@ -249,12 +249,45 @@ void upload_groups(orbc_ctx *c) {
     sync(c);
 }
 
+// Frees every device array of the store and forgets their pointers and the
+// capacities they were sized for (cell ranges, lists, block halos, bonds,
+// members), so that the next commit allocates all of them afresh instead of
+// reusing freed memory.
+void drop_store(orbc_ctx *c) {
+    voronoi_free(c);
+    c->mem.free_all();
+    c->live = false;
+    for (int b = 0; b < 2; ++b) { c->x[b] = nullptr; c->nrm[b] = nullptr; c->pq[b] = nullptr; }
+    c->v = c->o = c->v_alt = c->o_alt = nullptr;
+    c->f64 = c->t64 = c->f64_alt = c->t64_alt = nullptr;
+    c->f32 = c->t32 = c->f32_alt = c->t32_alt = nullptr;
+    c->e = c->w = c->fe = c->te = nullptr;
+    c->fe32 = c->te32 = nullptr;
+    c->f_ext = nullptr;
+    c->rb = nullptr;
+    c->anc = nullptr;
+    c->type = c->type_alt = nullptr;
+    c->id = c->id_alt = c->slot = c->slot_alt = c->inv = nullptr;
+    c->keys = c->vals = c->keys_alt = c->vals_alt = nullptr;
+    c->sort_temp = nullptr;
+    c->cstart = c->cend = nullptr;
+    c->ncell_cap = 0;
+    c->nbr = c->nnbr = nullptr;
+    c->kmax = 0;
+    c->halo_ext = c->halo_cnt = c->nbr16 = nullptr;
+    c->halo_nblk_cap = c->nbr16_cap = 0;
+    c->halo_ok = false;
+    c->bnd = nullptr; c->nbnd = nullptr; c->kbmax = 0;
+    c->bstart = c->bpartner = nullptr; c->btype_d = nullptr; c->nbonds = 0;
+    c->flags = nullptr; c->iflags = nullptr;
+    c->red = c->scal = c->bbox = c->stage = nullptr;
+    c->stagef = nullptr;
+    c->grp = c->grp_alt = nullptr;
+    c->sig_group = nullptr;
+}
+
 void commit(orbc_ctx *c) {
-    if (c->live) {
-        voronoi_free(c);
-        c->mem.free_all();
-        c->live = false;
-    }
+    if (c->live) drop_store(c);
     c->nl = c->sp[0].n;
     c->np = c->sp[1].n;
     c->n = c->nl + c->np;
@@ -700,11 +733,7 @@ namespace orbc {
 // state is dropped and the run counters reset; the caller uploads the state
 int orbc_set_config_internal(orbc_ctx *c, const orbc_config &cfg) {
     if (!config_valid(cfg) || cfg.device != c->cfg.device) return fail(c, ORBC_ERR_INVALID, "invalid configuration");
-    if (c->live) {
-        voronoi_free(c);
-        c->mem.free_all();
-        c->live = false;
-    }
+    if (c->live) drop_store(c);
     c->cfg = cfg;
     c->group_kbt.clear();
     for (auto &S : c->sp) S.group.clear();
@@ -735,8 +764,7 @@ extern "C" {
 int orbc_destroy(orbc_ctx *c) {
     if (!c) return ORBC_ERR_INVALID;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    voronoi_free(c);
-    c->mem.free_all();
+    drop_store(c);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_d) (void)hipHostFree(c->pinned_d);
@@ -797,9 +825,7 @@ int orbc_set_state(orbc_ctx *c, int species, long long n, const double *x, const
             c->need_rebuild = true;
             sync(c);
         } else if (c->live) {
-            c->live = false;   // counts, tags or types changed: re-commit on next use
-            voronoi_free(c);
-            c->mem.free_all();
+            drop_store(c);     // counts, tags or types changed: re-commit on next use
         }
         return (int)ORBC_OK;
     });

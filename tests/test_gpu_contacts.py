@@ -292,3 +292,30 @@ def test_nose_hoover_mass_fixed_at_first_use():
         sim.run(5)
         res.append((sim.get_state(orbc.LIPID)["v"], sim.observables()["zeta"]))
     assert np.array_equal(res[0][0], res[1][0]) and res[0][1] == res[1][1]
+
+
+@pytest.mark.parametrize("precision,tol", [(orbc.FP64, EPS_FP64), (orbc.MIXED, EPS_MIXED)])
+def test_reupload_other_system_into_live_context(precision, tol, oracle):
+    """A live context that receives a different system (other counts) drops
+    its device store and builds a new one: cell ranges, lists, block halos
+    and bond lists are allocated for the new system, not taken from the freed
+    one.  Three systems in a row, the last the same as the first; forces
+    against the oracle after each."""
+    sim = orbc.Simulation(precision=precision, block_halo=2)
+    for seed in (0, 1, 0):
+        s = contact_patch(seed=seed)
+        t = s["type"]
+        lip = np.where(t == 0)[0]
+        prot = np.where(t != 0)[0]
+        sim.set_state(orbc.LIPID, s["x"][lip], s["n"][lip], s["id"][lip], v=s["v"][lip], o=s["o"][lip])
+        sim.set_state(orbc.PROTEIN, s["x"][prot], s["n"][prot], s["id"][prot], v=s["v"][prot], o=s["o"][prot],
+                      type=t[prot])
+        bt, bi, bj = s["bonds"]
+        sim.set_bonds(bt, s["id"][bi], s["id"][bj])
+        sim.compute_pairwise(zero_first=True)
+        sim.compute_bonded()
+        f, tq = gather_forces(sim, lip, prot)
+        of, ot, _, _ = oracle.forces(s["x"], s["n"], s["type"], bonds=s["bonds"])
+        assert eps(f, of) < tol and eps(tq, ot) < tol
+        sim.run(3)
+        assert np.isfinite(gather_state(sim, lip, prot)["x"]).all()
