@@ -215,6 +215,19 @@ int orbc_delete_stray_lipids(orbc_ctx *ctx, double tolerance, long long *n_delet
 int orbc_morton_keys(orbc_ctx *ctx, long long n, const float *pts, double bsize, unsigned *keys);
 /* Stable LSD radix sort of (key, value) pairs on the GPU (bits <= 32). */
 int orbc_sort_pairs(orbc_ctx *ctx, long long n, unsigned *keys, unsigned *values, int bits);
+/* No counterpart in the reference: orbc_sort_pairs for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  keys and values are DEVICE arrays of n
+ * entries, sorted in place (stable, by the low `bits` bits of the key, the
+ * same order as orbc_sort_pairs).  Everything else the sort needs lives in
+ * a fixed per-device scratch of the library (about 9 MiB, held by every GPU
+ * the library runs on until the process ends); the call allocates nothing
+ * and frees nothing.  n <= ORBC_NOALLOC_SORT_MAX, else ORBC_ERR_INVALID.
+ * Asynchronous on the context's stream (orbc_synchronize before reading the
+ * results).  Calls that overlap on one GPU, from any context, race on that
+ * GPU's scratch: serialise them. */
+#define ORBC_NOALLOC_SORT_MAX (1LL << 20)
+int orbc_sort_pairs_noalloc(orbc_ctx *ctx, long long n, unsigned *keys, unsigned *values, int bits);
 /* KDTree::find_nearest / find_within replacements (src/kdtree.h:206-285):
  * nearest centroid per query and the centroid stencil within rmax. */
 int orbc_nearest(orbc_ctx *ctx, long long npts, const float *pts, long long nq, const float *q, int *idx);

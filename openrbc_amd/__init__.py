@@ -124,6 +124,7 @@ SIGNATURES = [
     ("orbc_delete_stray_lipids", C.c_int, [_vp, C.c_double, C.POINTER(C.c_longlong)]),
     ("orbc_morton_keys", C.c_int, [_vp, C.c_longlong, _f, C.c_double, _u]),
     ("orbc_sort_pairs", C.c_int, [_vp, C.c_longlong, _u, _u, C.c_int]),
+    ("orbc_sort_pairs_noalloc", C.c_int, [_vp, C.c_longlong, _vp, _vp, C.c_int]),
     ("orbc_nearest", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, _i]),
     ("orbc_within", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, C.c_float, C.c_int, _i, _i]),
     ("orbc_synchronize", C.c_int, [_vp]),
@@ -448,6 +449,12 @@ class Simulation:
         v = np.ascontiguousarray(values, dtype=np.uint32).copy()
         self._check(self.L.orbc_sort_pairs(self.h, len(k), _ptr(k, _u), _ptr(v, _u), int(bits)))
         return k, v
+
+    def sort_pairs_device(self, keys_ptr: int, values_ptr: int, n: int, bits=32):
+        """orbc_sort_pairs_noalloc on caller-owned device arrays (raw device
+        addresses); enqueued on the context's stream."""
+        self._check(self.L.orbc_sort_pairs_noalloc(self.h, int(n), C.c_void_p(keys_ptr), C.c_void_p(values_ptr),
+                                                   int(bits)))
 
     def nearest(self, pts, q):
         p = np.ascontiguousarray(pts, dtype=np.float32); qq = np.ascontiguousarray(q, dtype=np.float32)

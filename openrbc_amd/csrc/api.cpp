@@ -721,6 +721,14 @@ int orbc_create(const orbc_config *cfg, orbc_ctx **out) {
         delete c;
         return ORBC_ERR_HIP;
     }
+    // the fixed scratch of the allocation-free sort: its address on this
+    // device, looked up now with the device current (orbc_sort_pairs_noalloc
+    // reports a missing one)
+    try {
+        c->scratch = scratch_address(cfg_.device);
+    } catch (const std::exception &) {
+        c->scratch = nullptr;
+    }
     *out = c;
     return ORBC_OK;
 }

@@ -106,6 +106,7 @@ struct orbc_ctx {
     double *bbox = nullptr;      // 6 + 6*256
     double *stage = nullptr;     // 3n doubles, upload-order staging
     float *stagef = nullptr;
+    void *scratch = nullptr;     // this device's fixed sort scratch (orbc_sort_pairs_noalloc), set at create
     unsigned *pinned = nullptr;  // host-pinned flags
     double *pinned_d = nullptr;
 

@@ -10,6 +10,9 @@
 // scatter ranks equal digits inside each 64-lane wavefront with ballots (the
 // wave's lanes agreeing on all digit bits are its "peers"), then across the
 // block's four waves through LDS.  Deterministic: no atomics decide order.
+#include <mutex>
+#include <stdexcept>
+
 #include "orbc_internal.h"
 
 namespace orbc {
@@ -220,6 +223,50 @@ void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32
         alt = !alt;
     }
     *result_in_alt = alt;
+}
+
+// ---- fixed scratch of the allocation-free sort (orbc_sort_pairs_noalloc) ----
+// The sort's device memory beyond the caller's keys and values -- the
+// ping-pong key and value arrays and the histogram / scan temporaries for up
+// to NOALLOC_SORT_MAX pairs -- lives in this file-scope array instead of the
+// context's allocations.  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code the array's name is not a device
+// address: scratch_address() asks the runtime for this device's copy once,
+// with the device current, and callers pass that pointer to the kernels.
+namespace {
+constexpr size_t NOALLOC_PAIR_BYTES = 2 * sizeof(uint32_t) * (size_t)NOALLOC_SORT_MAX;   // k2, v2
+constexpr size_t NOALLOC_TEMP_BYTES = (size_t)1 << 20 | (size_t)4096;                      // >= radix_temp_bytes(max)
+constexpr size_t NOALLOC_BYTES = NOALLOC_PAIR_BYTES + NOALLOC_TEMP_BYTES;
+__device__ __align__(256) uint4 scratch[NOALLOC_BYTES / sizeof(uint4)];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};
+}  // namespace
+
+void *scratch_address(int device) {
+    if (radix_temp_bytes(NOALLOC_SORT_MAX) > NOALLOC_TEMP_BYTES)
+        throw std::runtime_error("fixed sort scratch smaller than the radix temporaries");
+    if (device < 0 || device >= 64) throw std::runtime_error("device ordinal out of range for the fixed scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) throw HipError(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+void sort_pairs_in_scratch(uint32_t *keys, uint32_t *vals, long long n, int bits, void *scr, hipStream_t s) {
+    if (n > NOALLOC_SORT_MAX) throw std::runtime_error("more pairs than the fixed scratch holds");
+    uint32_t *k2 = static_cast<uint32_t *>(scr);
+    uint32_t *v2 = k2 + NOALLOC_SORT_MAX;
+    void *temp = v2 + NOALLOC_SORT_MAX;
+    bool alt = false;
+    radix_sort_pairs(keys, vals, k2, v2, n, bits, temp, s, &alt);
+    if (alt) {
+        ORBC_HIP_CHECK(hipMemcpyAsync(keys, k2, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        ORBC_HIP_CHECK(hipMemcpyAsync(vals, v2, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    }
 }
 
 }  // namespace orbc

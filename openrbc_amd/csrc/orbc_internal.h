@@ -217,6 +217,12 @@ void upload_ff(const FFTab<double> &d, const FFTab<float> &f, hipStream_t s);
 size_t scan_temp_bytes(long long n);
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, long long n, void *temp, hipStream_t s, Gate gt = {});
 size_t radix_temp_bytes(long long n);
+// allocation-free sort (k_sort.hip): the device address of this device's
+// fixed scratch (looked up once per device, with it current) and the sort of
+// caller-owned device arrays through that scratch
+constexpr long long NOALLOC_SORT_MAX = 1LL << 20;
+void *scratch_address(int device);
+void sort_pairs_in_scratch(uint32_t *keys, uint32_t *vals, long long n, int bits, void *scr, hipStream_t s);
 void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, long long n,
                       int bits, void *temp, hipStream_t s, bool *result_in_alt, Gate gt = {});
 

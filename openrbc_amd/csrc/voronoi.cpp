@@ -363,6 +363,18 @@ int orbc_sort_pairs(orbc_ctx *c, long long n, unsigned *keys, unsigned *values, 
     });
 }
 
+int orbc_sort_pairs_noalloc(orbc_ctx *c, long long n, unsigned *keys, unsigned *values, int bits) {
+    return guarded(c, [&] {
+        if (n < 0 || bits < 1 || bits > 32 || (n > 0 && (!keys || !values))) return fail(c, ORBC_ERR_INVALID, "bad arguments");
+        if (n > NOALLOC_SORT_MAX)
+            return fail(c, ORBC_ERR_INVALID, "n exceeds ORBC_NOALLOC_SORT_MAX (1048576 pairs), the fixed scratch");
+        if (!c->scratch) return fail(c, ORBC_ERR_HIP, "the fixed sort scratch has no device address on this device");
+        if (n <= 1) return (int)ORBC_OK;
+        sort_pairs_in_scratch(keys, values, n, bits, c->scratch, c->stream);
+        return (int)ORBC_OK;
+    });
+}
+
 namespace {
 
 int point_queries(orbc_ctx *c, long long npts, const float *pts, long long nq, const float *q, bool nearest,
