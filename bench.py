@@ -37,6 +37,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 METRIC = "particle-timesteps/s on 1 MI355X, FP32-mixed vs FP64, at stated force error"
+# what the mixed path computes in (k_force.hip)
+MIXED_DTYPE = ("mixed: fp32 neighbour mirrors, selection, WCA/protein-protein/bond terms; anisotropic "
+               "lipid pair term in fp64 on the exact fp32-mirror separation; fp64 sums, state, integrator, "
+               "reductions")
 
 
 def algorithmic_bytes(precision: str, n: int, e_pairs: int, n_bonds: int) -> float:
@@ -114,31 +118,42 @@ def force_error(sim32, sim64, ids):
 
 def timed_run(sim, steps: int, warmup: int):
     """Returns wall seconds of `steps` steps, kernel timing, the observables
-    after the run, the rebuild count and the observables before it."""
+    after the run, the rebuild count and the observables before it.
+
+    The timed steps run as the library runs them (batches replayed from
+    captured hipGraphs).  HIP events around each fused kernel need the steps
+    launched one by one, so the kernel's duration comes from the `steps`
+    steps that follow the timed region, launched with events on the same
+    stream (same kernel, same arguments)."""
     sim.run(warmup)
     sim.synchronize()
-    sim.set_timing(True)
     obs0 = sim.observables()
     t0 = time.perf_counter()
     sim.run(steps)
     sim.synchronize()
     t1 = time.perf_counter()
-    tm = sim.timing()
     obs = sim.observables()
+    sim.set_timing(True)
+    sim.run(steps)
+    sim.synchronize()
+    tm = sim.timing()
     sim.set_timing(False)
     return t1 - t0, tm, obs, obs["rebuilds"] - obs0["rebuilds"], obs0
 
 
-def cpu_baseline(sysd, steps: int, n_lloyd: int, threads: int, timeout: int = 240):
+def cpu_baseline(sysd, steps: int, n_lloyd: int, threads: int, state=None, timeout: int = 300):
     """The OpenRBC reference itself (oracle/_ref, compiled in place from its
-    sources) on the same system, in a subprocess."""
+    sources) on the same system -- the minimised configuration the GPU runs
+    start from when `state` is given -- in a subprocess."""
     from oracle import REF_SO
     if not os.path.exists(REF_SO):
         return None
     import tempfile
     path = os.path.join(tempfile.gettempdir(), f"orbc_cpu_sample_{os.getpid()}.npz")
-    np.savez(path, lx=sysd["lipid_x"].astype(np.float32), ln=sysd["lipid_n"].astype(np.float32),
-             px=sysd["protein_x"].astype(np.float32), pn=sysd["protein_n"].astype(np.float32),
+    st = state if state is not None else dict(lx=sysd["lipid_x"], ln=sysd["lipid_n"], px=sysd["protein_x"],
+                                              pn=sysd["protein_n"])
+    np.savez(path, lx=st["lx"].astype(np.float32), ln=st["ln"].astype(np.float32),
+             px=st["px"].astype(np.float32), pn=st["pn"].astype(np.float32),
              pt=sysd["protein_type"], pid=sysd["protein_id"].astype(np.int32), bt=sysd["bond_type"],
              bi=sysd["bond_i"].astype(np.int32), bj=sysd["bond_j"].astype(np.int32))
     code = f"""
@@ -170,19 +185,57 @@ print(json.dumps(dict(seconds=t, n_final=nf.value)))
         except OSError:
             pass
     n = len(sysd["lipid_x"]) + len(sysd["protein_x"])
+    _, model, avail = host_cpus()
     return dict(value=n * steps / res["seconds"], unit="particle-timesteps/s", cores=threads, kind="reference",
                 sample=f"OpenRBC reference main loop (fused pairwise + bonded + verlet_langevin + Voronoi "
-                       f"updates every 2 steps, FP32 as shipped) on the same {n}-particle system: {steps} steps "
-                       f"after a {n_lloyd}-iteration Lloyd init, {threads} OpenMP threads",
-                seconds=res["seconds"])
+                       f"updates every 2 steps, FP32 as shipped, -O3 -ffast-math) on the same {n}-particle "
+                       f"system{' (the minimised configuration of the GPU runs)' if state is not None else ''}: "
+                       f"{steps} steps after the reference's {n_lloyd}-iteration Lloyd init, {threads} OpenMP "
+                       f"threads (OMP_PROC_BIND=close, OMP_PLACES=cores)",
+                cpu_model=model, logical_cpus_visible=avail, seconds=res["seconds"])
 
 
-def host_threads() -> int:
+def host_cpus():
+    """(threads to use, CPU model, logical CPUs this process may run on).
+    The threads are the host's CPU share as the environment states it
+    (OMP_NUM_THREADS, which the GPU host sets to its per-GPU share), else
+    every CPU of the affinity mask."""
     try:
-        n = len(os.sched_getaffinity(0))
+        avail = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else avail
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(threads, avail)), model, avail
+
+
+def traffic_profile(name: str, mesh: str):
+    """HBM bytes per launch of the fused step kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary, used only if that summary was
+    measured on the kernel sources of this build (their sha256 is recorded in
+    it); otherwise None."""
+    import hashlib
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_step_langevin_{name}_{mesh}.json")
+    if not os.path.exists(tpath):
+        return None, None
+    with open(tpath) as fh:
+        prof = json.load(fh)
+    h = hashlib.sha256()
+    for src in prof.get("sources", []):
+        with open(os.path.join(ROOT, src), "rb") as fh:
+            h.update(fh.read())
+    if not prof.get("sources") or h.hexdigest() != prof.get("sources_sha256"):
+        return None, os.path.relpath(tpath, ROOT) + " (measured on other kernel sources: not used)"
+    return prof.get("hbm_bytes_per_launch"), os.path.relpath(tpath, ROOT)
 
 
 def side_workload(args):
@@ -223,7 +276,7 @@ def side_workload(args):
     wall, tm, obs, rebuilds, obs0 = timed_run(sim, steps, min(args.warmup, 20))
     line = {"metric": METRIC, "value": n * steps / wall, "unit": "particle-timesteps/s", "n_gpus": 1,
             "steps": steps, "warmup": min(args.warmup, 20), "ms_per_step": 1e3 * wall / steps,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 forces / fp64 state",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": MIXED_DTYPE,
             "data": "synthetic", "config": {"workload": desc, "n_particles": n, "rebuilds": rebuilds,
                                             "temperature": obs["temperature"],
                                             "lipids_deleted": obs["lipids_deleted"],
@@ -241,8 +294,11 @@ def main():
     ap.add_argument("--minimize", type=int, default=1000)
     ap.add_argument("--skin", type=float, default=0.6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--cpu-lloyd", type=int, default=8)
+    ap.add_argument("--graphs", type=int, choices=[0, 1], default=1,
+                    help="1: orbc_run replays captured hipGraphs of its batches (default); 0: kernel by kernel")
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-lloyd", type=int, default=64, help="Lloyd iterations of the reference's own "
+                    "Voronoi init (src/openrbc.cpp:69-74 uses 64)")
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--mesh", choices=["example-large", "evans-fung"], default="example-large")
     ap.add_argument("--volume-strength", type=float, default=0.05,
@@ -288,6 +344,7 @@ def main():
     sim64.minimize(args.minimize)
     state = download(sim64)
     vol = dict(volume_target=-1.0 if args.volume_strength != 0 else 0.0, volume_strength=args.volume_strength)
+    vol["graphs"] = args.graphs
     sim32 = orbc.Simulation(precision=orbc.MIXED, seed=7, skin=args.skin, **vol)
     upload(sim32, sysd, state)
     sim64.close()
@@ -319,7 +376,9 @@ def main():
                          bytes_per_step=B, achieved_GBs=B / (kern_ms * 1e-3) / 1e9,
                          temperature=obs["temperature"], potential_per_particle=obs["potential"] / n,
                          energy_per_particle=dict(start=e0 / n, end=e1 / n, drift=(e1 - e0) / n),
-                         halo_blocks=obs["halo_blocks"], halo_overflow=obs["halo_overflow"])
+                         halo_blocks=obs["halo_blocks"], halo_overflow=obs["halo_overflow"],
+                         graph_replays=obs["graph_replays"] - obs0["graph_replays"],
+                         graph_captures=obs["graph_captures"])
         print(f"# {name}: {res[name]['value']:.4g} particle-steps/s, {res[name]['ms_per_step']:.3f} ms/step, "
               f"kernel {kern_ms:.3f} ms, rebuilds {rebuilds} ({tm['rebuild_ms']:.1f} ms), "
               f"T={obs['temperature']:.4f}, E/N {e0 / n:.5f} -> {e1 / n:.5f}, "
@@ -334,16 +393,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sysd, args.cpu_steps, args.cpu_lloyd, host_threads())
+        cpu = cpu_baseline(sysd, args.cpu_steps, args.cpu_lloyd, host_cpus()[0], state=state)
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     head = res.get("mixed") or res.get("fp64")
     hname = "mixed" if "mixed" in res else "fp64"
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_k_step_langevin_{hname}_{args.mesh}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    traffic, traffic_src = traffic_profile(hname, args.mesh)
     line = {
         "metric": METRIC,
         "value": head["value"],
@@ -355,7 +410,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 forces / fp64 state" if hname == "mixed" else "fp64",
+        "dtype": MIXED_DTYPE if hname == "mixed" else "fp64",
         "data": f"synthetic ({'Evans-Fung biconcave mesh, geodesic nu=34' if args.mesh == 'evans-fung' else 'OpenRBC example-large mesh'} "
                 "populated by init_rbc rules, seed 7; FP64-minimised; Maxwell velocities)",
         "config": {
@@ -366,6 +421,7 @@ def main():
             "n_particles": n, "n_bonds": nb, "pairs_within_cutoff": e_pairs, "skin": args.skin,
             "precision": hname, "mesh": args.mesh,
             "block_halos": {"blocks": head["halo_blocks"], "overflow": head["halo_overflow"]},
+            "graphs": {"replays_in_timed_region": head["graph_replays"], "captured": head["graph_captures"]},
         },
         "roofline": {
             "bound": "hbm",
@@ -374,9 +430,12 @@ def main():
             "unit": "GB/s",
             "frac": head["achieved_GBs"] * 1e9 / HBM_PEAK,
             "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": ("k_step_langevin_halo" if head["halo_blocks"] else "k_step_langevin")
                       + " (fused pair + bond + Langevin)",
             "kernel_ms": head["kernel_ms"],
+            "kernel_timing": f"HIP events around each fused kernel on the context stream, {args.steps} steps "
+                             f"launched one by one right after the timed region",
             "algorithmic_bytes_per_launch": head["bytes_per_step"],
         },
         "cpu_baseline": cpu,

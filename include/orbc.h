@@ -104,6 +104,10 @@ typedef struct orbc_config {
                                    (built at every neighbour rebuild).  1 (default): while a list lasts
                                    4 steps or more on average; 2: always; 0: never (every neighbour
                                    gathered from the 32-bit list).  Same forces either way. */
+    int graphs;                 /* orbc_run's Langevin batches: 1 (default) replay captured hipGraphs of
+                                   the whole timestep (Voronoi / volume work, the fused step kernel),
+                                   steps numbered on the device so one graph serves every batch of its
+                                   length and parity; 0: launched kernel by kernel.  Same results. */
 } orbc_config;
 
 typedef struct orbc_observables {
@@ -127,6 +131,8 @@ typedef struct orbc_observables {
     long long halo_blocks;          /* particle blocks of the fused steps with an LDS halo at the last
                                        rebuild (0 when no halos were built) */
     long long halo_overflow;        /* blocks whose neighbours did not fit (32-bit gathers) */
+    long long graph_captures;       /* batch graphs captured so far (orbc_run, cfg.graphs) */
+    long long graph_replays;        /* batch graph launches so far */
 } orbc_observables;
 
 void orbc_config_default(orbc_config *cfg);

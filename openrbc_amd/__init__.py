@@ -60,7 +60,7 @@ class Config(C.Structure):
         ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3), ("periodic", C.c_int * 3),
         ("skin", C.c_double), ("rebuild_period", C.c_int), ("neighbor_mode", C.c_int),
         ("thermostat", C.c_int), ("volume_target", C.c_double), ("volume_strength", C.c_double),
-        ("device", C.c_int), ("block_size", C.c_int), ("block_halo", C.c_int),
+        ("device", C.c_int), ("block_size", C.c_int), ("block_halo", C.c_int), ("graphs", C.c_int),
     ]
 
 
@@ -72,6 +72,7 @@ class Observables(C.Structure):
         ("anchor_lo", C.c_double * 3), ("anchor_spacing", C.c_double * 3), ("anchor_origin", C.c_double * 3),
         ("anchor_bits", C.c_int * 3), ("n_lipid", C.c_longlong), ("n_protein", C.c_longlong),
         ("lipids_deleted", C.c_longlong), ("halo_blocks", C.c_longlong), ("halo_overflow", C.c_longlong),
+        ("graph_captures", C.c_longlong), ("graph_replays", C.c_longlong),
     ]
 
 

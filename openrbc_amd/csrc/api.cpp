@@ -152,6 +152,9 @@ void alloc_store(orbc_ctx *c) {
     c->stage = m.alloc<double>(3 * n);
     c->stagef = m.alloc<float>(3 * n);
     c->f_ext = m.alloc<double>(3 * n);
+    c->step_dev = m.alloc<unsigned long long>(1);
+    c->anc_dev = m.alloc<Anchor>(1);
+    c->cg_dev = m.alloc<Grid>(1);
     ORBC_HIP_CHECK(hipMemsetAsync(c->flags, 0, 8 * sizeof(unsigned), c->stream));
     ORBC_HIP_CHECK(hipMemsetAsync(c->iflags, 0, 8 * sizeof(int), c->stream));
     ORBC_HIP_CHECK(hipMemsetAsync(c->scal, 0, 16 * sizeof(double), c->stream));
@@ -254,6 +257,7 @@ void upload_groups(orbc_ctx *c) {
 // members), so that the next commit allocates all of them afresh instead of
 // reusing freed memory.
 void drop_store(orbc_ctx *c) {
+    free_graphs(c);
     voronoi_free(c);
     c->mem.free_all();
     c->live = false;
@@ -284,6 +288,9 @@ void drop_store(orbc_ctx *c) {
     c->stagef = nullptr;
     c->grp = c->grp_alt = nullptr;
     c->sig_group = nullptr;
+    c->step_dev = nullptr;
+    c->anc_dev = nullptr;
+    c->cg_dev = nullptr;
 }
 
 void commit(orbc_ctx *c) {
@@ -510,6 +517,7 @@ void rebuild(orbc_ctx *c) {
     }
     if (!packed) throw std::runtime_error("cannot pack anchors");
     c->ag = ag;
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->anc_dev, &c->ag, sizeof(Anchor), hipMemcpyHostToDevice, s));
     // 4. Morton sort of the particles by grid cell
     launch_keys(c->x[c->cur], n, g, c->keys, c->vals, s);
     bool alt = false;
@@ -680,11 +688,13 @@ void orbc_config_default(orbc_config *cfg) {
     cfg->device = 0;
     cfg->block_size = 0;
     cfg->block_halo = 1;
+    cfg->graphs = 1;
 }
 
 static bool config_valid(const orbc_config &cfg_) {
     if (cfg_.dt <= 0 || cfg_.skin <= 0 || cfg_.kBT < 0 || cfg_.eta < 0) return false;
     if (cfg_.block_halo < 0 || cfg_.block_halo > 2) return false;
+    if (cfg_.graphs < 0 || cfg_.graphs > 1) return false;
     if (cfg_.block_size != 0 && cfg_.block_size != BLOCK) return false;   // the kernels are built for 256
     if (cfg_.voronoi_cell_size <= 0) return false;                         // lipids per Voronoi cell
     if (cfg_.precision != ORBC_FP64 && cfg_.precision != ORBC_MIXED) return false;
@@ -1040,6 +1050,7 @@ static StepArgs step_args(orbc_ctx *c) {
     a.has_bonds = c->nbonds > 0;
     a.f_ext = nullptr;
     a.anc_grid = c->ag;
+    a.anc_dev = c->anc_dev;
     a.dt = c->cfg.dt;
     langevin_coeffs(c->cfg, a.gamma, a.sigma);
     a.grp = c->grp; a.sig_group = c->sig_group;
@@ -1071,11 +1082,8 @@ static bool volume_on(const orbc_ctx *c) { return c->cfg.volume_target != 0.0 &&
 // Inside a batch every kernel of it is gated on the step's own skip
 // condition, and the fused step kernel applies the push; otherwise (to_f) the
 // push is added to the force accumulators.
-static void cell_prestep(orbc_ctx *c, long long step, bool gated, bool to_f, bool update_cells = true) {
-    if (gated) {
-        c->voro.gate.stop = c->iflags;
-        c->voro.gate.slot = c->flags + 4 + (int)((step + 2) % 3);
-    }
+static void cell_prestep(orbc_ctx *c, long long step, const Gate &gate, bool to_f, bool update_cells = true) {
+    c->voro.gate = gate;
     if (update_cells && step % c->cfg.freq_voronoi == 0) voronoi_update_dev(c);
     if (volume_on(c)) {
         if (to_f)
@@ -1117,6 +1125,106 @@ static void volume_begin(orbc_ctx *c) {
     c->vol_target_set = true;
 }
 
+// One step of a batch: the gated Voronoi / volume work and the fused step
+// kernel of step `step`.  step0 null: numbered on the host (k = step);
+// otherwise the step is *step0 + k on the device (captured graphs).
+static void enqueue_step(orbc_ctx *c, long long step, unsigned long long k, const unsigned long long *step0,
+                         hipEvent_t e0, hipEvent_t e1) {
+    const Gate gate{c->iflags, c->flags + 4, step0, k};
+    if (cells_in_run(c)) cell_prestep(c, step, gate, false);
+    StepArgs a = step_args(c);
+    a.step = k;
+    a.step0 = step0;
+    if (volume_on(c)) {
+        a.push = c->scal + 9;
+        a.cell_normal = c->voro.normals;
+        a.aff = c->voro.aff;
+    }
+    if (e0) ORBC_HIP_CHECK(hipEventRecord(e0, c->stream));
+    launch_step_langevin(a, c->mixed(), c->stream);
+    if (e1) ORBC_HIP_CHECK(hipEventRecord(e1, c->stream));
+    c->cur = 1 - c->cur;
+}
+
+// Everything a captured batch of `len` steps bakes into its kernel arguments
+// besides the device step counter: the step and Voronoi arguments of its
+// first step (pointers, constants, parity of the double buffers), the volume
+// target, the Voronoi phase of its first step and the timing events.
+static std::vector<unsigned char> batch_signature(orbc_ctx *c, int len, long long first_step) {
+    std::vector<unsigned char> sig;
+    auto put = [&](const void *p, size_t nb) {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        sig.insert(sig.end(), b, b + nb);
+    };
+    StepArgs a = step_args(c);
+    a.anc_grid = Anchor{};          // read from anc_dev by the kernels
+    a.step = 0;                     // numbered from the device counter
+    put(&a, sizeof(a));
+    const int head[8] = {len, c->cur, (int)(first_step % c->cfg.freq_voronoi), (int)cells_in_run(c),
+                         (int)volume_on(c), (int)c->voro.aff_valid, (int)c->timing, c->cfg.freq_voronoi};
+    put(head, sizeof(head));
+    const double vol[2] = {c->vol_target, c->cfg.volume_strength};
+    put(vol, sizeof(vol));
+    if (cells_in_run(c)) {
+        const auto &V = c->voro;
+        const void *vp[] = {V.aff, V.aff_alt, V.c, V.sums, V.normals, V.partial, V.cg_start, V.cg_end, V.cg_idx,
+                            V.cref, V.cdisp, c->x[0], c->x[1], c->nrm[0], c->nrm[1]};
+        put(vp, sizeof(vp));
+        put(&V.ncell, sizeof(V.ncell));
+    }
+    return sig;
+}
+
+
+// The captured batch of `len` steps starting at host step `first_step` with
+// the current parity, from the cache or captured now; nullptr when capture is
+// not available (the caller launches the steps one by one).
+static orbc_ctx::BatchGraph *batch_graph(orbc_ctx *c, int len, long long first_step) {
+    if (!c->graphs_ok) return nullptr;
+    std::vector<unsigned char> sig = batch_signature(c, len, first_step);
+    for (auto &g : c->graphs)
+        if (g.sig == sig) { g.last_use = ++c->graph_clock; return &g; }
+    if (c->graphs.size() >= 48) {
+        auto it = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                   [](const orbc_ctx::BatchGraph &x, const orbc_ctx::BatchGraph &y) { return x.last_use < y.last_use; });
+        if (it->exec) (void)hipGraphExecDestroy(it->exec);
+        if (it->graph) (void)hipGraphDestroy(it->graph);
+        c->graphs.erase(it);
+    }
+    orbc_ctx::BatchGraph g;
+    g.sig = std::move(sig);
+    const int cur0 = c->cur;
+    bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    c->capturing = ok;
+    if (ok) {
+        try {
+            for (int k = 0; k < len; ++k)
+                enqueue_step(c, first_step + k, (unsigned long long)k, c->step_dev, nullptr, nullptr);
+            launch_add_u64(c->step_dev, (unsigned long long)len, c->stream);
+        } catch (const HipError &) {
+            ok = false;
+        }
+        c->capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        ok = ok && e == hipSuccess && graph;
+        if (ok && hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) != hipSuccess) ok = false;
+        g.graph = graph;
+    }
+    c->cur = cur0;
+    if (!ok) {
+        (void)hipGetLastError();
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        c->graphs_ok = false;       // launched step by step from now on
+        return nullptr;
+    }
+    g.last_use = ++c->graph_clock;
+    c->graph_captures++;
+    c->graphs.push_back(std::move(g));
+    return &c->graphs.back();
+}
+
 static void run_langevin(orbc_ctx *c, long long nsteps) {
     long long remaining = nsteps;
     // the fused step keeps forces in registers: rebuilds inside the run leave
@@ -1131,7 +1239,6 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
     } scratch{c};
     c->ft_scratch = true;
     const bool vol = volume_on(c);
-    const bool cells = cells_in_run(c);
     const long long cperiod = cleanup_period(c);
     while (remaining > 0) {
         cleanup_if_due(c);
@@ -1148,22 +1255,52 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
             const long long left = (long long)c->list_life - c->since_build + margin;
             m = std::min<long long>(m, std::max<long long>(1, left));
         }
-        ORBC_HIP_CHECK(hipMemsetAsync(c->iflags + 1, 0, sizeof(int), c->stream));
+        launch_fill_u32(reinterpret_cast<uint32_t *>(c->iflags + 1), 0u, 1, c->stream);
         const int cur0 = c->cur;
-        if (c->timing) ev_pair(c, (size_t)m);
-        for (long long s = 0; s < m; ++s) {
-            if (cells) cell_prestep(c, c->step + s, true, false);
-            StepArgs a = step_args(c);
-            a.step = (unsigned long long)(c->step + s);
-            if (vol) {
-                a.push = c->scal + 9;
-                a.cell_normal = c->voro.normals;
-                a.aff = c->voro.aff;
+        // captured graphs: the batch as segments of 32, 16, ..., 1 steps;
+        // otherwise (or with kernel timing on: events recorded inside a graph
+        // cannot be read back here) the steps launched one by one
+        struct Seg { orbc_ctx::BatchGraph *g; long long off; int len; };
+        std::vector<Seg> segs;
+        if (c->cfg.graphs && c->graphs_ok && !c->timing) {
+            // capture every length for this parity and the next step's at once
+            // (a run needs them all soon; capturing costs host time while the
+            // GPU waits, so it is paid once instead of inside later batches)
+            const long long miss_before = c->graph_captures;
+            (void)batch_graph(c, 32, c->step);
+            if (c->graph_captures != miss_before && c->graphs_ok) {
+                for (int par = 0; par < 2 && c->graphs_ok; ++par) {
+                    c->cur = par ? 1 - cur0 : cur0;
+                    for (int len = 32; len >= 1 && c->graphs_ok; len >>= 1) (void)batch_graph(c, len, c->step + par);
+                }
+                c->cur = cur0;
             }
-            if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s], c->stream));
-            launch_step_langevin(a, c->mixed(), c->stream);
-            if (c->timing) ORBC_HIP_CHECK(hipEventRecord(c->ev[2 * s + 1], c->stream));
-            c->cur = 1 - c->cur;
+            launch_set_u64(c->step_dev, (unsigned long long)c->step, c->stream);
+            long long off = 0;
+            for (int len = 32; len >= 1 && c->graphs_ok; len >>= 1) {
+                if (m - off < len) continue;
+                orbc_ctx::BatchGraph *g = batch_graph(c, len, c->step + off);   // at this segment's parity
+                if (!g) break;
+                segs.push_back({g, off, len});
+                off += len;
+                if (len & 1) c->cur = 1 - c->cur;
+            }
+            c->cur = cur0;
+            if (c->graphs_ok) {
+                for (const Seg &sg : segs) {
+                    ORBC_HIP_CHECK(hipGraphLaunch(sg.g->exec, c->stream));
+                    c->graph_replays++;
+                    if (sg.len & 1) c->cur = 1 - c->cur;
+                }
+            } else {
+                segs.clear();
+            }
+        }
+        if (segs.empty()) {
+            if (c->timing) ev_pair(c, (size_t)m);
+            for (long long s = 0; s < m; ++s)
+                enqueue_step(c, c->step + s, (unsigned long long)(c->step + s), nullptr,
+                             c->timing ? c->ev[2 * s] : nullptr, c->timing ? c->ev[2 * s + 1] : nullptr);
         }
         int fl[2];
         ORBC_HIP_CHECK(hipMemcpyAsync(fl, c->iflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
@@ -1171,13 +1308,15 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
         ORBC_HIP_CHECK(hipMemcpyAsync(&bad, c->flags + 1, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
         sync(c);
         const long long done = fl[1];
-        if (c->timing)
-            for (long long s = 0; s < done; ++s) {
+        if (c->timing) {
+            auto add = [&](hipEvent_t e0, hipEvent_t e1) {
                 float ms = 0;
-                ORBC_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]));
+                ORBC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
                 c->t_step_ms += ms;
                 c->n_step_launches++;
-            }
+            };
+            for (long long s = 0; s < done; ++s) add(c->ev[2 * s], c->ev[2 * s + 1]);
+        }
         c->cur = (cur0 + (int)(done & 1)) & 1;
         c->step += done;
         c->since_build += done;
@@ -1237,7 +1376,7 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 clear_ft(c, false);
                 ForceArgs fa = force_args(c);
                 launch_forces(fa, c->mixed(), false, true, c->nbonds > 0, c->stream);
-                if (cells) cell_prestep(c, c->step, false, true, update);
+                if (cells) cell_prestep(c, c->step, Gate{}, true, update);
             };
             if (!c->nh_started) {
                 forces(false);
@@ -1339,6 +1478,8 @@ int orbc_get_observables(orbc_ctx *c, orbc_observables *obs) {
         obs->lipids_deleted = c->deleted;
         obs->step = c->step;
         obs->rebuilds = c->rebuilds;
+        obs->graph_captures = c->graph_captures;
+        obs->graph_replays = c->graph_replays;
         if (c->halo_ok && c->n > 0) {
             const long long nb = (c->n + BLOCK - 1) / BLOCK;
             std::vector<uint32_t> hc(nb);

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <list>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -107,6 +108,25 @@ struct orbc_ctx {
     double *stage = nullptr;     // 3n doubles, upload-order staging
     float *stagef = nullptr;
     void *scratch = nullptr;     // this device's fixed sort scratch (orbc_sort_pairs_noalloc), set at create
+    // captured batches of orbc_run's Langevin loop (cfg.graphs): a graph of
+    // `len` steps numbered from the device counter *step_dev, found by the
+    // signature of everything else its kernels bake in
+    struct BatchGraph {
+        std::vector<unsigned char> sig;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long last_use = 0;
+    };
+    std::list<BatchGraph> graphs;     // a list: segments of one batch keep pointers to their entries
+    bool graphs_ok = true;            // capture works here (cleared at the first failure)
+    unsigned long long graph_clock = 0;
+    long long graph_captures = 0, graph_replays = 0;
+    unsigned long long *step_dev = nullptr;
+    // device copies of the anchor grid and the centroid grid, rewritten at
+    // every neighbour rebuild; kernels of captured batches read them there
+    Anchor *anc_dev = nullptr;
+    Grid *cg_dev = nullptr;
+    bool capturing = false;           // a batch is being captured (voro_args points kernels at cg_dev)
     unsigned *pinned = nullptr;  // host-pinned flags
     double *pinned_d = nullptr;
 
@@ -172,6 +192,14 @@ void ensure_live(orbc_ctx *c);
 void ensure_list(orbc_ctx *c);
 void rebuild(orbc_ctx *c);
 void voronoi_free(orbc_ctx *c);
+// destroys the captured batch graphs (they hold the store's device pointers)
+inline void free_graphs(orbc_ctx *c) {
+    for (auto &g : c->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    c->graphs.clear();
+}
 void voronoi_rebuild_index(orbc_ctx *c);
 void voronoi_update_dev(orbc_ctx *c);
 // stray-lipid cleanup (src/cleanup.h:29-91) with the current cells; removes

@@ -779,21 +779,21 @@ __device__ __forceinline__ PQ halo_stage(const StepArgs &a, long long blk, bool 
 
 // pair and bond forces of active particle i from the staged halo (or, for an
 // overflowed block, gathered one neighbour at a time)
-__device__ __forceinline__ void halo_accumulate(const StepArgs &a, long long i, const PQ &qi, uint32_t hcnt,
+__device__ __forceinline__ void halo_accumulate(const StepArgs &a, const Anchor &ag, long long i, const PQ &qi, uint32_t hcnt,
                                                 const HaloLDS &L, Acc<double> &acc) {
     const int ti = __float_as_int(qi.n.w);
-    const AnchorI ai = unpack(__float_as_uint(qi.x.w), a.anc_grid);
+    const AnchorI ai = unpack(__float_as_uint(qi.x.w), ag);
     const int nn = (int)a.nnbr[i];
     const int crow = ti * NT;
     if (hcnt != HALO_OVERFLOW) {
-        range_halo(nn, a.stride, a.nbr16 + i, L.x, L.n, L.pp, crow, a.anc_grid, ai, qi.x, qi.n, acc);
+        range_halo(nn, a.stride, a.nbr16 + i, L.x, L.n, L.pp, crow, ag, ai, qi.x, qi.n, acc);
     } else {
         for (int k = 0; k < nn; ++k) {
             const PQ q = a.pq_in[a.nbr[(long long)k * a.stride + i]];
-            pair_rec(a.anc_grid, ai, qi.x, qi.n, q.x, q.n, L.pp[crow + __float_as_int(q.n.w)], acc);
+            pair_rec(ag, ai, qi.x, qi.n, q.x, q.n, L.pp[crow + __float_as_int(q.n.w)], acc);
         }
     }
-    if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, a.anc_grid, ai, qi.x, L, acc);
+    if (a.has_bonds) bonds_mixed<false>(i, a.stride, a.pq_in, a.bnd, a.nbnd, ag, ai, qi.x, L, acc);
 }
 
 // FP64 block halo: positions and directors of the halo particles (48 bytes,
@@ -829,12 +829,11 @@ __device__ __forceinline__ void halo_stage64(const StepArgs &a, long long blk, b
     __syncthreads();
 }
 
-__device__ __forceinline__ void halo_accumulate64(const StepArgs &a, long long i, uint32_t hcnt,
+__device__ __forceinline__ void halo_accumulate64(const StepArgs &a, const Anchor &ag, long long i, uint32_t hcnt,
                                                   const HaloLDS64 &L, const FFS<double> &t, Acc<double> &acc) {
     const double xi = a.x_in[3 * i], yi = a.x_in[3 * i + 1], zi = a.x_in[3 * i + 2];
     const double nix = a.n_in[3 * i], niy = a.n_in[3 * i + 1], niz = a.n_in[3 * i + 2];
     const int ti = a.type[i];
-    const Anchor &ag = a.anc_grid;
     const bool periodic = ag.L[0] > 0 || ag.L[1] > 0 || ag.L[2] > 0;
     const int nn = (int)a.nnbr[i];
     if (hcnt != HALO_OVERFLOW) {
@@ -867,7 +866,11 @@ __device__ __forceinline__ void halo_accumulate64(const StepArgs &a, long long i
 
 template <bool MIXED, bool HALO>
 __device__ __forceinline__ void step_langevin(const StepArgs &a) {
-    const int k3 = (int)(a.step % 3ull);
+    // the anchor grid from its device copy (rewritten at every neighbour
+    // rebuild, so a captured batch stays valid), a uniform load at the top
+    const Anchor ag = *a.anc_dev;
+    const unsigned long long step = (a.step0 ? *a.step0 : 0ull) + a.step;
+    const int k3 = (int)(step % 3ull);
     // skin flag of this step's input positions; no block of this launch writes it
     if (*a.stop || a.dslot[(k3 + 2) % 3] != 0u) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *a.stop = 1;
@@ -903,21 +906,21 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         if constexpr (HALO && MIXED) {
             ti = __float_as_int(qi.n.w);
             wi = __float_as_uint(qi.x.w);
-            halo_accumulate(a, i, qi, hcnt, hl, acc);
+            halo_accumulate(a, ag, i, qi, hcnt, hl, acc);
         } else if constexpr (HALO) {
-            halo_accumulate64(a, i, hcnt, hl64, tab, acc);
+            halo_accumulate64(a, ag, i, hcnt, hl64, tab, acc);
             ti = a.type[i];
             wi = a.anc[i];
         } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
             const float4 xi4 = qi.x, ni4 = qi.n;
-            accumulate_mixed<false>(i, a.stride, a.pq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, a.anc_grid, xi4,
+            accumulate_mixed<false>(i, a.stride, a.pq_in, a.nbr, a.nnbr, a.bnd, a.nbnd, ag, xi4,
                                     ni4, tab, acc, true, a.has_bonds != 0);
             ti = __float_as_int(ni4.w);
             wi = __float_as_uint(xi4.w);
         } else {
             accumulate_fp64<false>(i, a.stride, a.x_in, a.n_in, a.type, a.nbr, a.nnbr, a.bnd, a.nbnd,
-                                   a.anc_grid, tab, acc, true, a.has_bonds != 0);
+                                   ag, tab, acc, true, a.has_bonds != 0);
             ti = a.type[i];
             wi = a.anc[i];
         }
@@ -943,7 +946,7 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         const double gs = rsqrt(gx * gx + gy * gy + gz * gz);
         gx *= gs; gy *= gs; gz *= gs;
         // translation with uniform [-1,1) noise (src/integrate_langevin.h:282-296)
-        uint32_t c0 = a.id[i], c1 = (uint32_t)a.step, c2 = (uint32_t)(a.step >> 32), c3 = 0u;
+        uint32_t c0 = a.id[i], c1 = (uint32_t)step, c2 = (uint32_t)(step >> 32), c3 = 0u;
         philox4x32_10(c0, c1, c2, c3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
         const double u = 4.656612873077392578125e-10;
         const double rx = (double)c0 * u - 1.0, ry = (double)c1 * u - 1.0, rz = (double)c2 * u - 1.0;
@@ -960,10 +963,10 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         a.v[3 * i] = vx; a.v[3 * i + 1] = vy; a.v[3 * i + 2] = vz;
         a.o[3 * i] = ox; a.o[3 * i + 1] = oy; a.o[3 * i + 2] = oz;
         // FP32 mirror with the build-time anchor, and the skin check
-        const AnchorI ai = unpack(wi, a.anc_grid);
-        const float r0 = __double2float_rn(px - anchor_pos(a.anc_grid, 0, ai.x));
-        const float r1 = __double2float_rn(py - anchor_pos(a.anc_grid, 1, ai.y));
-        const float r2 = __double2float_rn(pz - anchor_pos(a.anc_grid, 2, ai.z));
+        const AnchorI ai = unpack(wi, ag);
+        const float r0 = __double2float_rn(px - anchor_pos(ag, 0, ai.x));
+        const float r1 = __double2float_rn(py - anchor_pos(ag, 1, ai.y));
+        const float r2 = __double2float_rn(pz - anchor_pos(ag, 2, ai.z));
         if (MIXED) {
             PQ q;
             q.x = make_float4(r0, r1, r2, __uint_as_float(wi));
@@ -1020,10 +1023,10 @@ __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, fl
         int ti;
         Acc<double> acc;
         if constexpr (HALO && MIXED) {
-            halo_accumulate(a, i, qh, hcnt, hl, acc);
+            halo_accumulate(a, a.anc_grid, i, qh, hcnt, hl, acc);
             ti = __float_as_int(qh.n.w);
         } else if constexpr (HALO) {
-            halo_accumulate64(a, i, hcnt, hl64, tab, acc);
+            halo_accumulate64(a, a.anc_grid, i, hcnt, hl64, tab, acc);
             ti = a.type[i];
         } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];

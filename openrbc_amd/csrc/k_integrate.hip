@@ -420,6 +420,18 @@ void launch_gather_u16(long long n, const uint32_t *slot, const uint16_t *src, u
     if (n > 0) hipLaunchKernelGGL(k_gather_u16, dim3(nblk(n)), dim3(BLOCK), 0, s, n, slot, src, dst);
 }
 
+namespace {
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+__global__ void k_add_u64(unsigned long long *p, unsigned long long v) { *p += v; }
+}  // namespace
+
+void launch_set_u64(unsigned long long *p, unsigned long long v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, p, v);
+}
+void launch_add_u64(unsigned long long *p, unsigned long long v, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(1), 0, s, p, v);
+}
+
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s, Gate gt) {
     if (n <= 0) return;
     long long nb = nblk(n);

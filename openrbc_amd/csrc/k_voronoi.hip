@@ -80,7 +80,7 @@ __device__ inline float centroid_drift(const VoroArgs &a) {
 // nearest centroid (lowest index on equal squared distance), grid rings outward
 __device__ int nearest_one(const VoroArgs &a, float p0, float p1, float p2) {
     const float drift = centroid_drift(a);
-    const Grid &g = a.cg;
+    const Grid g = *a.cg_dev;
     const int cx = gcoord(p0, g, 0), cy = gcoord(p1, g, 1), cz = gcoord(p2, g, 2);
     float best = INFINITY;
     int bi = -1;
@@ -120,7 +120,7 @@ __device__ int nearest_one(const VoroArgs &a, float p0, float p1, float p2) {
 // box are scanned (typically 1-4 instead of 27).  The radius carries a
 // relative margin so FP32 rounding of the box ends cannot drop a candidate.
 __device__ int nearest_warm(const VoroArgs &a, float p0, float p1, float p2, int k0) {
-    const Grid &g = a.cg;
+    const Grid g = *a.cg_dev;
     float best = normsq_unfused(p0 - a.centroids[3 * k0], p1 - a.centroids[3 * k0 + 1], p2 - a.centroids[3 * k0 + 2]);
     int bi = k0;
     // a centroid as near as k0 was binned within r + drift of p
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(BLOCK) void k_nearest_wave(VoroArgs a, const double
     const int w = threadIdx.x >> 6, lane = __lane_id();
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     const bool act = i < nq;
-    const Grid &g = a.cg;
+    const Grid g = *a.cg_dev;
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, best = INFINITY, r = 0.f;
     int bi = -1, k0 = -1;
     if (act) {
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_within(VoroArgs a, const float *q, lo
                                                   int *counts, int *out) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= nq) return;
-    const Grid &g = a.cg;
+    const Grid g = *a.cg_dev;
     const float p0 = q[3 * i], p1 = q[3 * i + 1], p2 = q[3 * i + 2];
     const float r2 = rmax * rmax;
     const int R = (int)ceilf(rmax / (float)g.cs) + 1;

@@ -112,11 +112,22 @@ struct Box {
 // work of step k is enqueued before step k's fused kernel; when *stop is set or
 // step k-1 raised its skin flag (*slot), step k will be skipped, and every
 // kernel of its work returns at once.  A default Gate is always open.
+// Skip gate of the work queued before a batched step (see StepArgs::dslot):
+// closed when the run has stopped or when the step's input positions violate
+// the skin (slot (step + 2) % 3).  The step is *step0 + k when step0 is set
+// (steps numbered on the device, so a captured graph serves any start step),
+// else k.
 struct Gate {
     const int *stop = nullptr;
-    const unsigned *slot = nullptr;
+    const unsigned *slots = nullptr;
+    const unsigned long long *step0 = nullptr;
+    unsigned long long k = 0;
 };
-__device__ inline bool gate_closed(const Gate &g) { return g.stop && (*g.stop != 0 || *g.slot != 0u); }
+__device__ inline bool gate_closed(const Gate &g) {
+    if (!g.stop) return false;
+    const unsigned long long st = (g.step0 ? *g.step0 : 0ull) + g.k;
+    return *g.stop != 0 || g.slots[(st + 2ull) % 3ull] != 0u;
+}
 
 // FP32 mirror of one particle for the mixed path, one 32-byte record so a
 // neighbour gather touches one cache line: x = position offset from the
@@ -179,6 +190,12 @@ struct StepArgs {
     const uint16_t *grp;
     const double *sig_group;
     unsigned long long seed, step;
+    // step0 set: the step is *step0 + step (device-side numbering of a
+    // captured batch); null: the step is `step`
+    const unsigned long long *step0;
+    // the anchor grid as the step kernels read it: a device copy of anc_grid,
+    // rewritten at every neighbour rebuild (so captured batches stay valid)
+    const Anchor *anc_dev;
     // skin criterion on the device: step k reads slot (k+2)%3 (displacement of
     // its input positions), writes slot k%3 (of its output) and clears slot
     // (k+1)%3 for step k+1.  A step whose input violates the skin, or that
@@ -342,6 +359,9 @@ void launch_scatter_slots_f(long long n, const uint32_t *slot, const float *src3
 void launch_scatter_slots1(long long n, const uint32_t *slot, const double *src, double *dst, hipStream_t s);
 void launch_gather_slots(long long n, const uint32_t *slot, const double *src3, double *dst3, hipStream_t s);
 void launch_gather_slots_f(long long n, const uint32_t *slot, const double *src3, float *dst3, hipStream_t s);
+// single device counters of the batched run: *p = v, *p += v
+void launch_set_u64(unsigned long long *p, unsigned long long v, hipStream_t s);
+void launch_add_u64(unsigned long long *p, unsigned long long v, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s, Gate gt = {});
 void launch_gather_u16(long long n, const uint32_t *slot, const uint16_t *src, uint16_t *dst, hipStream_t s);
 
@@ -359,6 +379,8 @@ struct VoroArgs {
     Grid cg;
     const uint32_t *cg_start, *cg_end, *cg_idx;
     Gate gate;               // inside a batched run: the step this work precedes may be skipped
+    const Grid *cg_dev;      // the centroid grid as the kernels read it: a device copy of cg, rewritten
+                             // whenever the grid is rebuilt (so captured batches stay valid)
     // The centroid grid is built at neighbour rebuilds only: cref holds the
     // centroids as binned, *cdisp the largest squared drift of a centroid since
     // (as float bits).  Queries widen their reach by that drift, so they stay

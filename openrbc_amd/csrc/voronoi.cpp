@@ -75,6 +75,7 @@ VoroArgs voro_args(orbc_ctx *c) {
     a.gate = c->voro.gate;
     a.cref = c->voro.cref;
     a.cdisp = c->voro.cdisp;
+    a.cg_dev = c->cg_dev;
     return a;
 }
 
@@ -86,6 +87,7 @@ void index_centroids(orbc_ctx *c) {
         hi[d] = c->grid.lo[d] + c->grid.csd[d] * c->grid.dim[d];
     }
     V.cg = make_grid(lo, hi, kCentroidCell);
+    ORBC_HIP_CHECK(hipMemcpyAsync(c->cg_dev, &V.cg, sizeof(Grid), hipMemcpyHostToDevice, c->stream));
     if (V.cg.ncell > V.cg_cap) {
         c->mem.release(V.cg_start); c->mem.release(V.cg_end);
         V.cg_cap = (long long)(1.25 * (double)V.cg.ncell) + 16;
@@ -400,6 +402,10 @@ int point_queries(orbc_ctx *c, long long npts, const float *pts, long long nq, c
     VoroArgs a{};
     a.ncell = npts; a.centroids = dp; a.cg = g; a.cg_start = st; a.cg_end = en; a.cg_idx = idx_d;
     std::vector<void *> extra;
+    Grid *gd = c->mem.alloc<Grid>(1);
+    extra.push_back(gd);
+    ORBC_HIP_CHECK(hipMemcpyAsync(gd, &a.cg, sizeof(Grid), hipMemcpyHostToDevice, c->stream));
+    a.cg_dev = gd;
     if (nearest) {
         int32_t *o = c->mem.alloc<int32_t>(nq);
         extra.push_back(o);
