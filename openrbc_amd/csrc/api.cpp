@@ -153,6 +153,8 @@ void alloc_store(orbc_ctx *c) {
     c->stagef = m.alloc<float>(3 * n);
     c->f_ext = m.alloc<double>(3 * n);
     c->step_dev = m.alloc<unsigned long long>(1);
+    c->bud2 = m.alloc<float>(n);
+    ORBC_HIP_CHECK(hipMemsetAsync(c->bud2, 0, sizeof(float) * n, c->stream));
     c->anc_dev = m.alloc<Anchor>(1);
     c->cg_dev = m.alloc<Grid>(1);
     ORBC_HIP_CHECK(hipMemsetAsync(c->flags, 0, 8 * sizeof(unsigned), c->stream));
@@ -289,6 +291,7 @@ void drop_store(orbc_ctx *c) {
     c->grp = c->grp_alt = nullptr;
     c->sig_group = nullptr;
     c->step_dev = nullptr;
+    c->bud2 = nullptr;
     c->anc_dev = nullptr;
     c->cg_dev = nullptr;
 }
@@ -377,7 +380,22 @@ void set_cutoffs(orbc_ctx *c, double cs) {
         c->rings[a] = rings[a];
         if ((c->types_present >> a & 1u) && rings[a] >= 2) c->far_rings = true;
     }
-    set_list_cutoffs(cut2, rings, c->stream);
+    // skin budgets of isolated particles (k_budget): types that search one
+    // ring of cells for their list and whose longest cutoff leaves less than a
+    // skin to the cell edge (lipids, membrane proteins) search two rings, so
+    // a drifting one gets a budget of up to (2 h - rc) / 2; other types keep skin/2
+    float pcut[36];
+    int brings[NT];
+    for (int a = 0; a < NT; ++a) {
+        double rmax = 0;
+        for (int b = 0; b < NT; ++b) {
+            const double rc = (c->types_present >> b & 1u) ? ff_max_cut(a, b) : 0.0;
+            pcut[a * NT + b] = (float)rc;
+            rmax = std::max(rmax, rc);
+        }
+        brings[a] = (rings[a] == 1 && rmax > 0 && cs - rmax < 2 * skin) ? 2 : 0;
+    }
+    set_list_cutoffs(cut2, rings, pcut, brings, c->stream);
 }
 
 // gathers every per-particle array of the store through perm (n_out
@@ -416,6 +434,18 @@ static void permute_store(orbc_ctx *c, const uint32_t *perm, long long n_out, bo
         std::swap(c->f32, c->f32_alt); std::swap(c->t32, c->t32_alt);
         std::swap(c->f64, c->f64_alt); std::swap(c->t64, c->t64_alt);
     }
+    std::swap(c->type, c->type_alt); std::swap(c->id, c->id_alt); std::swap(c->slot, c->slot_alt);
+    if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
+    if (c->grp) std::swap(c->grp, c->grp_alt);
+}
+
+// The pointer pairs a rebuild's permutation swaps (and the double-buffer
+// parity it flips), without moving any data: applied twice it is the
+// identity.  Used to capture the batch graphs of the store as it will be
+// after the next rebuild.
+static void toggle_store_parity(orbc_ctx *c) {
+    c->cur = 1 - c->cur;
+    std::swap(c->v, c->v_alt); std::swap(c->o, c->o_alt);
     std::swap(c->type, c->type_alt); std::swap(c->id, c->id_alt); std::swap(c->slot, c->slot_alt);
     if (c->voro.aff) std::swap(c->voro.aff, c->voro.aff_alt);
     if (c->grp) std::swap(c->grp, c->grp_alt);
@@ -582,6 +612,9 @@ void rebuild(orbc_ctx *c) {
         launch_halo(n, c->stride, c->nbr, c->nnbr, c->type, c->halo_ext, c->halo_cnt, c->nbr16, s);
         c->halo_ok = true;
     }
+    // 7c. skin budgets of the particles left without list entries (open boxes)
+    if (!any_per)
+        launch_budget(n, c->x[c->cur], c->type, c->nnbr, g, c->cstart, c->cend, (float)(0.5 * c->cfg.skin), c->bud2, s);
     // 8. bond lists
     if (c->nbonds > 0)
         launch_bondlist(n, c->stride, c->slot, c->inv, c->bstart, c->bpartner, c->btype_d, c->bnd, c->nbnd, s);
@@ -1051,6 +1084,9 @@ static StepArgs step_args(orbc_ctx *c) {
     a.f_ext = nullptr;
     a.anc_grid = c->ag;
     a.anc_dev = c->anc_dev;
+    bool per = false;
+    for (int d = 0; d < 3; ++d) per = per || c->cfg.periodic[d];
+    a.bud2 = per ? nullptr : c->bud2;
     a.dt = c->cfg.dt;
     langevin_coeffs(c->cfg, a.gamma, a.sigma);
     a.grp = c->grp; a.sig_group = c->sig_group;
@@ -1263,15 +1299,24 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
         struct Seg { orbc_ctx::BatchGraph *g; long long off; int len; };
         std::vector<Seg> segs;
         if (c->cfg.graphs && c->graphs_ok && !c->timing) {
-            // capture every length for this parity and the next step's at once
-            // (a run needs them all soon; capturing costs host time while the
-            // GPU waits, so it is paid once instead of inside later batches)
+            // capture every length for this parity and the next step's, for
+            // the store as it is and as it will be after the next rebuild, at
+            // once: a run needs them all soon, and capturing costs host time
+            // while the GPU waits, so it is paid once instead of inside later
+            // batches (a rebuild swaps the permuted arrays' buffers)
             const long long miss_before = c->graph_captures;
             (void)batch_graph(c, 32, c->step);
             if (c->graph_captures != miss_before && c->graphs_ok) {
-                for (int par = 0; par < 2 && c->graphs_ok; ++par) {
-                    c->cur = par ? 1 - cur0 : cur0;
-                    for (int len = 32; len >= 1 && c->graphs_ok; len >>= 1) (void)batch_graph(c, len, c->step + par);
+                for (int rb = 0; rb < 2 && c->graphs_ok; ++rb) {
+                    if (rb) toggle_store_parity(c);
+                    const int base = c->cur;
+                    for (int par = 0; par < 2 && c->graphs_ok; ++par) {
+                        c->cur = par ? 1 - base : base;
+                        for (int len = 32; len >= 1 && c->graphs_ok; len >>= 1)
+                            (void)batch_graph(c, len, c->step + par);
+                    }
+                    c->cur = base;
+                    if (rb) toggle_store_parity(c);
                 }
                 c->cur = cur0;
             }

@@ -122,6 +122,7 @@ struct orbc_ctx {
     unsigned long long graph_clock = 0;
     long long graph_captures = 0, graph_replays = 0;
     unsigned long long *step_dev = nullptr;
+    float *bud2 = nullptr;            // squared skin budget of isolated particles (k_budget), 0: skin/2
     // device copies of the anchor grid and the centroid grid, rewritten at
     // every neighbour rebuild; kernels of captured batches read them there
     Anchor *anc_dev = nullptr;

@@ -23,6 +23,8 @@ __device__ inline float normsq_unfused(float a, float b, float c) {
 
 __constant__ float c_list_cut2[36];   // (rc(a,b) + skin)^2, <= 0 if a,b never interact
 __constant__ int c_rings[NT];         // grid rings a type searches
+__constant__ float c_pair_cut[36];    // interaction cutoff of a type pair (0: none)
+__constant__ int c_budget_rings[NT];  // rings searched for the skin budget of an isolated particle (0: none)
 
 __global__ __launch_bounds__(BLOCK) void k_bbox_partial(const double *x, long long n, double *partial) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -707,9 +709,71 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
     }
 }
 
+// Skin budget of isolated particles.  The list stays valid while every pair
+// not in it keeps d_i + d_j < r_ij(build) - rc_ij.  A particle with list
+// entries gets the usual budget skin/2 (its unlisted pairs start at rc + skin
+// or more).  A particle with no entries at all (a lipid ejected from a
+// vesicle, drifting away) gets b_i = min_j (r_ij - rc_ij) / 2 over the
+// particles j it could interact with in the cells within `R` rings, capped by
+// (R h - max_j rc_ij) / 2 for everything beyond them (h the cell edge): then
+// any two particles keep d_i + d_j <= r_ij - rc_ij as long as each stays within
+// its own budget, and one drifting lipid no longer triggers a rebuild every
+// step.  Stores b_i^2 (FP32, minus a rounding margin) where it exceeds
+// (skin/2)^2, else 0 ("skin/2").  Open boxes only (the caller skips periodic
+// ones).
+__global__ __launch_bounds__(BLOCK) void k_budget(long long n, const double *x, const uint8_t *type,
+                                                  const uint32_t *nnbr, Grid g, const uint32_t *cstart,
+                                                  const uint32_t *cend, float half_skin, float *bud2) {
+    const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int ti = type[i];
+    const int R = c_budget_rings[ti];
+    float out = 0.f;
+    if (R > 0 && nnbr[i] == 0) {
+        const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+        double cmax = 0.0;
+        for (int tj = 0; tj < NT; ++tj) cmax = fmax(cmax, (double)c_pair_cut[ti * NT + tj]);
+        const double h = fmin(g.csd[0], fmin(g.csd[1], g.csd[2]));
+        double best = 0.5 * (R * h - cmax);
+        const int cx = cell_coord(xi, g.lo[0], g.csd[0], g.dim[0]);
+        const int cy = cell_coord(yi, g.lo[1], g.csd[1], g.dim[1]);
+        const int cz = cell_coord(zi, g.lo[2], g.csd[2], g.dim[2]);
+        for (int dz = -R; dz <= R; ++dz) {
+            int zz;
+            if (!cell_step(g, 2, cz, dz, zz)) continue;
+            for (int dy = -R; dy <= R; ++dy) {
+                int yy;
+                if (!cell_step(g, 1, cy, dy, yy)) continue;
+                for (int dx = -R; dx <= R; ++dx) {
+                    int xx;
+                    if (!cell_step(g, 0, cx, dx, xx)) continue;
+                    const long long cl = cell_lin(xx, yy, zz, g);
+                    for (uint32_t j = cstart[cl]; j < cend[cl]; ++j) {
+                        if (j == (uint32_t)i) continue;
+                        const double c = c_pair_cut[ti * NT + type[j]];
+                        if (!(c > 0.0)) continue;
+                        const double ex = xi - x[3 * j], ey = yi - x[3 * j + 1], ez = zi - x[3 * j + 2];
+                        best = fmin(best, 0.5 * (sqrt(ex * ex + ey * ey + ez * ez) - c));
+                    }
+                }
+            }
+        }
+        const float b = (float)best - 1e-4f;
+        if (b > half_skin) out = b * b;
+    }
+    bud2[i] = out;
+}
+
 inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
 }  // namespace
+
+void launch_budget(long long n, const double *x, const uint8_t *type, const uint32_t *nnbr, const Grid &g,
+                   const uint32_t *cstart, const uint32_t *cend, float half_skin, float *bud2, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_budget, dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, type, nnbr, g, cstart, cend, half_skin,
+                       bud2);
+}
 
 void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
                  uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, hipStream_t s) {
@@ -723,10 +787,15 @@ void launch_disp(long long n, const double *x, const uint32_t *anc, const float 
     hipLaunchKernelGGL(k_disp, dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, anc, rb, g, disp_max);
 }
 
-void set_list_cutoffs(const float *cut2_list, const int *rings, hipStream_t s) {
+void set_list_cutoffs(const float *cut2_list, const int *rings, const float *pair_cut, const int *budget_rings,
+                      hipStream_t s) {
     ORBC_HIP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_list_cut2), cut2_list, sizeof(float) * 36, 0,
                                           hipMemcpyHostToDevice, s));
     ORBC_HIP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rings), rings, sizeof(int) * NT, 0,
+                                          hipMemcpyHostToDevice, s));
+    ORBC_HIP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_pair_cut), pair_cut, sizeof(float) * 36, 0,
+                                          hipMemcpyHostToDevice, s));
+    ORBC_HIP_CHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_budget_rings), budget_rings, sizeof(int) * NT, 0,
                                           hipMemcpyHostToDevice, s));
 }
 

@@ -978,8 +978,11 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         bad = !(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(gx));
         if (bad) d2 = 3.0e38f;
     }
-    // one atomic per wave, and only when some lane moved past skin/2
-    const unsigned long long over = __ballot(d2 > a.half_skin2);
+    // one atomic per wave, and only when some lane moved past its budget:
+    // skin/2, or for a particle without list entries the budget k_budget gave it
+    bool past = d2 > a.half_skin2;
+    if (past && a.bud2 && !bad && a.nnbr[i] == 0) past = d2 > a.bud2[i];
+    const unsigned long long over = __ballot(past);
     if (over && __lane_id() == (int)__ffsll((long long)over) - 1) atomicOr(a.dslot + k3, 1u);
     if (bad) atomicOr(a.bad, 1u);
 }

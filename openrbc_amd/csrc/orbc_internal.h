@@ -203,6 +203,7 @@ struct StepArgs {
     // batch of steps and rebuild once after it.
     unsigned *dslot;           // 3 flags: some particle moved more than skin/2 since build
     float half_skin2;          // (skin/2)^2
+    const float *bud2;         // squared skin budget of isolated particles (k_budget; 0: skin/2), or null
     int *stop;                 // sticky: a step was skipped
     int *done;                 // steps executed (incremented by block 0)
     unsigned *bad;             // nonzero if a non-finite value appeared
@@ -297,7 +298,11 @@ struct FarScratch {
 };
 void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool far, const FarScratch &fs,
                   hipStream_t s);
-void set_list_cutoffs(const float *cut2_list, const int *rings, hipStream_t s);
+void set_list_cutoffs(const float *cut2_list, const int *rings, const float *pair_cut, const int *budget_rings,
+                      hipStream_t s);
+// skin budget of the particles without list entries (k_build.hip, k_budget)
+void launch_budget(long long n, const double *x, const uint8_t *type, const uint32_t *nnbr, const Grid &g,
+                   const uint32_t *cstart, const uint32_t *cend, float half_skin, float *bud2, hipStream_t s);
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g,
                  unsigned *disp_max, hipStream_t s);
 void launch_bondlist(long long n, long long stride, const uint32_t *slot, const uint32_t *inv,
