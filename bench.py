@@ -273,6 +273,12 @@ def side_workload(args):
         # Voronoi cells as in the reference loop: updated every freq_voronoi steps, and the
         # stray-lipid cleanup every freq_cleanup steps removes lipids ejected from the vesicles
         sim.voronoi_init(0, 4)
+    tuning = None
+    if args.tune:
+        tuning = sim.autotune(trial_steps=args.tune_steps or min(steps, 100), warm_steps=min(args.warmup, 20),
+                              table=args.tune_table)
+        tuning["knobs"] = sim.tuning()
+        print(f"# autotune: {tuning}", file=sys.stderr, flush=True)
     wall, tm, obs, rebuilds, obs0 = timed_run(sim, steps, min(args.warmup, 20))
     line = {"metric": METRIC, "value": n * steps / wall, "unit": "particle-timesteps/s", "n_gpus": 1,
             "steps": steps, "warmup": min(args.warmup, 20), "ms_per_step": 1e3 * wall / steps,
@@ -282,7 +288,8 @@ def side_workload(args):
                                             "lipids_deleted": obs["lipids_deleted"],
                                             "step_kernel_ms": tm["step_kernel_ms"] / max(1, tm["step_launches"]),
                                             "block_halos": {"blocks": obs["halo_blocks"],
-                                                            "overflow": obs["halo_overflow"]}}}
+                                                            "overflow": obs["halo_overflow"]}},
+            "tuning": tuning}
     print(json.dumps(line), flush=True)
 
 
@@ -294,6 +301,12 @@ def main():
     ap.add_argument("--minimize", type=int, default=1000)
     ap.add_argument("--skin", type=float, default=0.6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", type=int, choices=[0, 1], default=1,
+                    help="1: orbc_autotune before the timed region (untimed set-up), trials of --warmup + "
+                         "--tune-steps steps")
+    ap.add_argument("--tune-steps", type=int, default=0,
+                    help="timed steps per autotune trial (0: --steps, at most 100)")
+    ap.add_argument("--tune-table", default=None, help="append the autotuner's trial table (JSON lines) here")
     ap.add_argument("--graphs", type=int, choices=[0, 1], default=1,
                     help="1: orbc_run replays captured hipGraphs of its batches (default); 0: kernel by kernel")
     ap.add_argument("--cpu-steps", type=int, default=100)
@@ -363,9 +376,19 @@ def main():
     print(f"# minimised: {err_min} pairs={e_pairs}", file=sys.stderr, flush=True)
 
     res = {}
+    tuning = {}
     for name, sim in (("mixed", sim32), ("fp64", sim64)):
         if args.precision not in ("both", name):
             continue
+        if args.tune:
+            # untimed set-up, like the minimisation: orbc_autotune on this system (state restored)
+            t_t = time.perf_counter()
+            # trials shaped like the timed run: warmup steps, then the timed steps
+            tuning[name] = sim.autotune(trial_steps=args.tune_steps or min(args.steps, 100), warm_steps=args.warmup,
+                                        table=args.tune_table)
+            tuning[name]["knobs"] = sim.tuning()
+            print(f"# {name} autotune ({time.perf_counter() - t_t:.1f} s): {tuning[name]}", file=sys.stderr,
+                  flush=True)
         wall, tm, obs, rebuilds, obs0 = timed_run(sim, args.steps, args.warmup)
         e0 = obs0["kinetic"] + obs0["potential"]
         e1 = obs["kinetic"] + obs["potential"]
@@ -439,6 +462,7 @@ def main():
             "algorithmic_bytes_per_launch": head["bytes_per_step"],
         },
         "cpu_baseline": cpu,
+        "tuning": tuning or None,
         "force_error": {"minimised": err_min, "after_dynamics": err_dyn},
     }
     if "mixed" in res and "fp64" in res:

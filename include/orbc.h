@@ -271,6 +271,33 @@ int orbc_synchronize(orbc_ctx *ctx);
 /* Kernel timing of orbc_run with HIP events on the context's stream:
  * accumulated time of the fused step kernel and of neighbour rebuilds
  * (measurement hook; the reference's Timers, src/timer.h:52-69). */
+/* Run-time autotuner of orbc_run (no reference counterpart).  Measures the
+ * knobs that decide the main loop's speed on the uploaded system itself: the
+ * Verlet skin, block halos of the fused step (block_halo 0/1/2), the edge of
+ * the centroid grid behind the nearest-centroid partition (with Voronoi
+ * cells), and the steps launched per host synchronisation.  Every trial
+ * restores the saved state, changes one knob and times orbc_run over
+ * trial_steps steps (wall clock) after warm_steps untimed ones (< 0:
+ * trial_steps / 4) -- the shape of the run the caller is about to do; knobs
+ * are tuned one after another in that order.  The state (positions,
+ * velocities, directors, step counter, cells, Nose-Hoover friction) is
+ * restored at the end and the best configuration applied; the stray-lipid
+ * cleanup does not run inside the trials.  With table_path, one JSON line of
+ * the trials and the choice is appended to that file. */
+typedef struct orbc_tuning {
+    double skin;
+    int block_halo;
+    double centroid_cell;        /* edge of the centroid grid (default 8) */
+    int batch;                   /* steps launched per host synchronisation, 1..32 (default 32) */
+    double ms_per_step;          /* measured, chosen configuration (orbc_autotune) */
+    double ms_per_step_start;    /* measured, configuration before tuning (orbc_autotune) */
+    int trials;
+} orbc_tuning;
+int orbc_autotune(orbc_ctx *ctx, int warm_steps, int trial_steps, const char *table_path, orbc_tuning *out);
+/* apply a configuration (e.g. a line of a stored table); takes effect at the next rebuild */
+int orbc_set_tuning(orbc_ctx *ctx, const orbc_tuning *t);
+int orbc_get_tuning(const orbc_ctx *ctx, orbc_tuning *t);
+
 int orbc_set_timing(orbc_ctx *ctx, int on);
 int orbc_get_timing(orbc_ctx *ctx, double *step_kernel_ms, long long *step_launches, double *rebuild_ms,
                     long long *rebuilds);

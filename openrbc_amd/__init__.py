@@ -64,6 +64,15 @@ class Config(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """orbc_tuning: the knobs orbc_autotune chooses."""
+
+    _fields_ = [
+        ("skin", C.c_double), ("block_halo", C.c_int), ("centroid_cell", C.c_double), ("batch", C.c_int),
+        ("ms_per_step", C.c_double), ("ms_per_step_start", C.c_double), ("trials", C.c_int),
+    ]
+
+
 class Observables(C.Structure):
     _fields_ = [
         ("kinetic", C.c_double), ("temperature", C.c_double), ("potential", C.c_double),
@@ -126,6 +135,9 @@ SIGNATURES = [
     ("orbc_morton_keys", C.c_int, [_vp, C.c_longlong, _f, C.c_double, _u]),
     ("orbc_sort_pairs", C.c_int, [_vp, C.c_longlong, _u, _u, C.c_int]),
     ("orbc_sort_pairs_noalloc", C.c_int, [_vp, C.c_longlong, _vp, _vp, C.c_int]),
+    ("orbc_autotune", C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p, C.POINTER(Tuning)]),
+    ("orbc_set_tuning", C.c_int, [_vp, C.POINTER(Tuning)]),
+    ("orbc_get_tuning", C.c_int, [_vp, C.POINTER(Tuning)]),
     ("orbc_nearest", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, _i]),
     ("orbc_within", C.c_int, [_vp, C.c_longlong, _f, C.c_longlong, _f, C.c_float, C.c_int, _i, _i]),
     ("orbc_synchronize", C.c_int, [_vp]),
@@ -383,6 +395,24 @@ class Simulation:
         self._check(self.L.orbc_get_pairs(self.h, m, _ptr(a, _u), _ptr(b, _u), C.byref(cnt)))
         p = np.stack([a[:m], b[:m]], 1).astype(np.int64)
         return p[np.lexsort((p[:, 1], p[:, 0]))]
+
+    def autotune(self, trial_steps: int = 40, table: str | None = None, warm_steps: int = -1) -> dict:
+        """orbc_autotune: measure skin, block halos, centroid-grid edge and batch
+        length on this system over runs of warm_steps + trial_steps steps and
+        apply the fastest (state restored)."""
+        t = Tuning()
+        self._check(self.L.orbc_autotune(self.h, int(warm_steps), int(trial_steps),
+                                         table.encode() if table else None, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, skin: float, block_halo: int, centroid_cell: float, batch: int):
+        t = Tuning(skin=skin, block_halo=block_halo, centroid_cell=centroid_cell, batch=batch)
+        self._check(self.L.orbc_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(self.L.orbc_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k in ("skin", "block_halo", "centroid_cell", "batch")}
 
     def synchronize(self):
         self._check(self.L.orbc_synchronize(self.h))

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "liborbc_hip.so")
 SOURCES = ["k_sort.hip", "k_build.hip", "k_force.hip", "k_integrate.hip", "k_voronoi.hip",
-           "ff.cpp", "api.cpp", "voronoi.cpp", "systems.cpp", "io.cpp"]
+           "ff.cpp", "api.cpp", "voronoi.cpp", "systems.cpp", "io.cpp", "tune.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ORBC_OFFLOAD_ARCH", "gfx950")
 

@@ -1144,6 +1144,7 @@ static long long cleanup_period(const orbc_ctx *c) {
 }
 
 static void cleanup_if_due(orbc_ctx *c) {
+    if (c->tuning) return;
     const long long L = cleanup_period(c);
     if (L <= 0 || c->step % L != 0 || c->last_cleanup == c->step) return;
     delete_stray_lipids(c, c->cfg.stray_tolerance);

@@ -141,6 +141,8 @@ struct orbc_ctx {
     double nh_q = 0.0;           // Nose-Hoover mass resolved at its first use (cfg.Q == 0: 0.01 N then)
     unsigned types_present = 0;  // bit t set if some particle has type t
     int batch = 32;              // fused steps launched per host synchronisation
+    double cg_edge = 8.0;        // edge of the centroid grid (nearest-centroid partition)
+    bool tuning = false;         // inside orbc_autotune's trials: no stray-lipid cleanup
     // kernel timing (orbc_set_timing)
     bool timing = false;
     double t_step_ms = 0, t_rebuild_ms = 0;

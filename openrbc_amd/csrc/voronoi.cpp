@@ -18,11 +18,6 @@ using namespace orbc;
 
 namespace {
 
-#ifndef ORBC_CENTROID_CELL
-#define ORBC_CENTROID_CELL 8.0
-#endif
-constexpr double kCentroidCell = ORBC_CENTROID_CELL;   // centroid-grid edge (~2 x centroid spacing)
-
 int ceil_log2(long long v) {
     int b = 0;
     while ((1LL << b) < v) ++b;
@@ -86,7 +81,7 @@ void index_centroids(orbc_ctx *c) {
         lo[d] = c->grid.lo[d];
         hi[d] = c->grid.lo[d] + c->grid.csd[d] * c->grid.dim[d];
     }
-    V.cg = make_grid(lo, hi, kCentroidCell);
+    V.cg = make_grid(lo, hi, c->cg_edge);
     ORBC_HIP_CHECK(hipMemcpyAsync(c->cg_dev, &V.cg, sizeof(Grid), hipMemcpyHostToDevice, c->stream));
     if (V.cg.ncell > V.cg_cap) {
         c->mem.release(V.cg_start); c->mem.release(V.cg_end);
