@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <numeric>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -757,6 +758,10 @@ int orbc_create(const orbc_config *cfg, orbc_ctx **out) {
         delete c;
         return ORBC_ERR_HIP;
     }
+    // test hook: the Nose-Hoover run loop one step at a time (the batched loop
+    // must give the same result)
+    const char *nh_env = std::getenv("ORBC_NH_PER_STEP");
+    c->nh_per_step = nh_env && nh_env[0] == '1';
     ff_fill(c->ffd, c->fff);
     try {
         upload_ff(c->ffd, c->fff, c->stream);
@@ -1390,6 +1395,67 @@ static void run_langevin(orbc_ctx *c, long long nsteps) {
     c->moved = true;
 }
 
+// Nose-Hoover (src/openrbc.cpp:176-243 with FUSED_INTEGRATOR) in batches:
+// step k = first half (which also flags slot k % 3 when a particle moved past
+// its skin budget) + zeta update + fused forces and second half + zeta update,
+// every kernel skipping once the run has stopped.  The second half of a step
+// whose first half set the flag stops the run instead; the host then rebuilds
+// between that step's halves, as the per-step loop does, finishes the step
+// and launches the next batch.
+static void run_nh_batched(orbc_ctx *c, long long nsteps, double Q) {
+    const double nkT = 1.5 * (double)c->n * c->cfg.kBT;    // no cleanup here (no cells): N is fixed
+    const long long nb = reduce_blocks(c->n);
+    const float half2 = (float)(0.25 * c->cfg.skin * c->cfg.skin);
+    bool per = false;
+    for (int d = 0; d < 3; ++d) per = per || c->cfg.periodic[d];
+    launch_fill_u32(c->flags + 4, 0u, 3, c->stream);
+    long long remaining = nsteps;
+    while (remaining > 0) {
+        long long m = std::min<long long>(c->batch, remaining);
+        if (c->list_life > 0)
+            m = std::min<long long>(m, std::max<long long>(1, (long long)c->list_life - c->since_build + 4));
+        launch_fill_u32(reinterpret_cast<uint32_t *>(c->iflags), 0u, 2, c->stream);
+        for (long long s = 0; s < m; ++s) {
+            const long long k = c->step + s;
+            IntegArgs a = integ_args(c);
+            a.rb = c->rb;
+            a.stop = c->iflags;
+            a.skin_slot = c->flags + 4 + (int)(k % 3);
+            a.skin_clear = c->flags + 4 + (int)((k + 1) % 3);
+            a.bud2 = per ? nullptr : c->bud2;
+            a.nnbr = c->nnbr;
+            a.half_skin2 = half2;
+            launch_integrate(ORBC_NH_INITIAL_BOUNCE_CLEAR, a, c->mixed(), c->stream);
+            launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream, c->iflags);
+            StepArgs sa = step_args(c);
+            sa.step = (unsigned long long)k;
+            sa.nh_gate = 1;
+            launch_step_nh_second(sa, c->mixed(), c->f32, c->t32, c->f64, c->t64, c->red, c->scal, c->stream);
+            launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream, c->iflags);
+        }
+        int fl[2];
+        ORBC_HIP_CHECK(hipMemcpyAsync(fl, c->iflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
+        sync(c);
+        const long long done = fl[1];
+        c->step += done;
+        c->since_build += done;
+        remaining -= done;
+        if (fl[0]) {
+            // the first half of step c->step ran and found the list spent
+            c->since_build++;
+            c->list_life = c->list_life > 0 ? 0.75 * c->list_life + 0.25 * (double)c->since_build
+                                            : (double)c->since_build;
+            rebuild(c);
+            launch_step_nh_second(step_args(c), c->mixed(), c->f32, c->t32, c->f64, c->t64, c->red, c->scal,
+                                  c->stream);
+            launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
+            c->step++;
+            remaining--;
+        }
+    }
+    c->moved = true;
+}
+
 int orbc_run(orbc_ctx *c, long long nsteps) {
     return guarded(c, [&] {
         if (nsteps < 0) return fail(c, ORBC_ERR_INVALID, "nsteps < 0");
@@ -1430,6 +1496,14 @@ int orbc_run(orbc_ctx *c, long long nsteps) {
                 launch_integrate(ORBC_NH_POST_TORQUE_FINAL, a, c->mixed(), c->stream);
                 launch_zeta_update(c->red, nb, c->scal, c->cfg.dt, Q, nkT, c->stream);
                 c->nh_started = true;
+            }
+            // batched: the skin check runs on the device between the halves and
+            // the host synchronises once per batch (the per-step loop below is
+            // kept for runs with Voronoi cells, whose volume push joins the
+            // unfused forces, and for fixed rebuild periods)
+            if (!cells && c->cfg.rebuild_period <= 0 && !c->nh_per_step) {
+                run_nh_batched(c, nsteps, Q);
+                nsteps = 0;
             }
             for (long long s = 0; s < nsteps; ++s) {
                 a = integ_args(c);

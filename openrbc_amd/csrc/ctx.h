@@ -143,6 +143,7 @@ struct orbc_ctx {
     int batch = 32;              // fused steps launched per host synchronisation
     double cg_edge = 8.0;        // edge of the centroid grid (nearest-centroid partition)
     bool tuning = false;         // inside orbc_autotune's trials: no stray-lipid cleanup
+    bool nh_per_step = false;    // Nose-Hoover run loop one step at a time (ORBC_NH_PER_STEP=1, tests)
     // kernel timing (orbc_set_timing)
     bool timing = false;
     double t_step_ms = 0, t_rebuild_ms = 0;

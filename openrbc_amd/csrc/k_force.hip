@@ -1006,6 +1006,16 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin64_halo(StepArgs a) { st
 template <bool MIXED, bool HALO>
 __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, float *t32, double *f64, double *t64,
                                                double *ke_partial, const double *zeta_p) {
+    if (a.nh_gate) {
+        // batched run: the first half of this step found a particle past its
+        // budget (slot step % 3), or an earlier step stopped: nothing to do
+        const int k3 = (int)(a.step % 3ull);
+        if (*a.stop || a.dslot[k3] != 0u) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) *a.stop = 1;
+            return;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.done, 1);
+    }
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
     __shared__ HaloLDS hl;
     __shared__ HaloLDS64 hl64;

@@ -128,9 +128,12 @@ __global__ __launch_bounds__(BLOCK) void k_langevin(IntegArgs a) {
 // verlet_initial_bounce_clearforce_update, src/integrate_nh.h:187-230
 template <typename F>
 __global__ __launch_bounds__(BLOCK) void k_nh_first(IntegArgs a) {
+    if (a.stop && *a.stop) return;          // a batched run stopped before this step (whole grid)
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (a.skin_clear && i == 0) *a.skin_clear = 0u;
     double ke = 0.0;
     float d2 = 0.f;
+    bool past = false;
     if (i < a.n) {
         const int ty = a.type[i];
         const double m = c_mass[ty];
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(BLOCK) void k_nh_first(IntegArgs a) {
         const double z[3] = {0, 0, 0};
         store_ft<F>(a, i, z, z);
         write_mirror(a, i, x, n, ty);
-        if (a.disp_max) {
+        if (a.disp_max || a.skin_slot) {
             const uint32_t w = a.anc[i];
             const Anchor &g = a.anc_grid;
             const long long A[3] = {(long long)(w & g.mx), (long long)((w >> g.shy) & g.my), (long long)(w >> g.shz)};
@@ -163,7 +166,13 @@ __global__ __launch_bounds__(BLOCK) void k_nh_first(IntegArgs a) {
                 d2 += e * e;
             }
             if (!isfinite(d2)) d2 = 3.0e38f;
+            past = d2 > a.half_skin2;
+            if (past && a.bud2 && d2 < 3.0e38f && a.nnbr[i] == 0) past = d2 > a.bud2[i];
         }
+    }
+    if (a.skin_slot) {
+        const unsigned long long over = __ballot(past);
+        if (over && __lane_id() == (int)__ffsll((long long)over) - 1) atomicOr(a.skin_slot, 1u);
     }
     if (a.disp_max) {
         // one atomic per wave, only where the wave raises the maximum (as k_disp)
@@ -280,7 +289,8 @@ __global__ __launch_bounds__(BLOCK) void k_sum_partials(const double *partial, l
 // zeta += dt/2 / Q (KE - 3/2 N kT), the functor destructors
 // src/integrate_nh.h:182-185,241-244
 __global__ __launch_bounds__(BLOCK) void k_zeta_update(const double *partial, long long nb, double *zeta, double dt,
-                                                       double Q, double nkT15) {
+                                                       double Q, double nkT15, const int *stop) {
+    if (stop && *stop) return;
     double s = 0.0;
     for (long long k = threadIdx.x; k < nb; k += BLOCK) s += partial[k];
 #pragma unroll
@@ -388,8 +398,8 @@ void launch_sum_partials(const double *partial, long long nb, double *out, hipSt
 }
 
 void launch_zeta_update(const double *partial, long long nb, double *zeta, double dt, double Q, double nkT15,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_zeta_update, dim3(1), dim3(BLOCK), 0, s, partial, nb, zeta, dt, Q, nkT15);
+                        hipStream_t s, const int *stop) {
+    hipLaunchKernelGGL(k_zeta_update, dim3(1), dim3(BLOCK), 0, s, partial, nb, zeta, dt, Q, nkT15, stop);
 }
 
 void launch_sum_array(const double *a, long long n, double *partial, hipStream_t s) {

@@ -206,6 +206,7 @@ struct StepArgs {
     const float *bud2;         // squared skin budget of isolated particles (k_budget; 0: skin/2), or null
     int *stop;                 // sticky: a step was skipped
     int *done;                 // steps executed (incremented by block 0)
+    int nh_gate;               // Nose-Hoover second half in a batch: skip on *stop or dslot[step % 3]
     unsigned *bad;             // nonzero if a non-finite value appeared
     int swizzle;               // XCD-aware block remap
 };
@@ -349,13 +350,22 @@ struct IntegArgs {
     // is folded into *disp_max (float bits, atomicMax)
     const float *rb;
     unsigned *disp_max;
+    // batched Nose-Hoover run (orbc_run): stop set -> the kernel does nothing;
+    // skin_slot set -> the first half ors 1 into it where a particle moved
+    // past its budget (skin/2, or bud2 for particles without list entries) and
+    // clears skin_clear for the step after
+    const int *stop;
+    unsigned *skin_slot, *skin_clear;
+    const float *bud2;
+    const uint32_t *nnbr;
+    float half_skin2;
 };
 void launch_integrate(int kernel, const IntegArgs &a, bool mixed, hipStream_t s);
 void launch_minimize(const IntegArgs &a, bool mixed, hipStream_t s);
 void launch_reduce_ke(const double *v, const uint8_t *type, long long n, double *partial, hipStream_t s);
 void launch_sum_partials(const double *partial, long long nblocks, double *out, hipStream_t s);
 void launch_zeta_update(const double *partial, long long nblocks, double *zeta, double dt, double Q,
-                        double nkT15, hipStream_t s);
+                        double nkT15, hipStream_t s, const int *stop = nullptr);
 void launch_sum_array(const double *a, long long n, double *partial, hipStream_t s);
 long long reduce_blocks(long long n);
 void launch_check_finite(const double *x, long long n3, unsigned *bad, hipStream_t s);

@@ -107,3 +107,30 @@ def test_graph_timing_falls_back_to_launches():
         sim.close()
     for k in res[0]:
         assert np.array_equal(res[0][k], res[1][k]), k
+
+
+@pytest.mark.parametrize("precision", [orbc.MIXED, orbc.FP64])
+@pytest.mark.parametrize("skin", [0.05, 0.4])
+def test_nose_hoover_batches_equal_per_step_loop(rbc, precision, skin, monkeypatch):
+    """The Nose-Hoover run in batches (skin check on the device between the
+    two halves, a rebuild between the halves of the step that spends the list,
+    one host synchronisation per batch) against the loop that synchronises
+    every step (ORBC_NH_PER_STEP=1): same kernels in the same order, so the
+    states are bit-identical, through rebuilds and over several orbc_run calls."""
+    print(f"NH batches: precision {precision}, skin {skin}", flush=True)
+    res = []
+    for per_step in ("0", "1"):
+        monkeypatch.setenv("ORBC_NH_PER_STEP", per_step)
+        sim = _sim(rbc, 1, precision, seed=5, skin=skin, thermostat=orbc.RUN_NOSE_HOOVER, kBT=0.22)
+        sim.minimize(10)
+        for n in (9, 30, 1):
+            sim.run(n)
+        res.append((_states(sim), sim.observables()))
+        sim.close()
+    (a, oa), (b, ob) = res
+    for sp in a:
+        for k in a[sp]:
+            assert np.array_equal(a[sp][k], b[sp][k]), (sp, k)
+    assert oa["zeta"] == ob["zeta"] and oa["step"] == ob["step"] == 40 and oa["rebuilds"] == ob["rebuilds"]
+    if skin < 0.1:
+        assert oa["rebuilds"] > 3
