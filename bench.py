@@ -243,7 +243,25 @@ def side_workload(args):
     the same metric on that workload (mixed precision)."""
     import openrbc_amd as orbc
     t0 = time.perf_counter()
-    if args.workload == "ensemble":
+    if args.workload == "ensemble" and args.vesicles == "mesh":
+        e = orbc.vesicle_ensemble_mesh(n_members=64, seed=0)
+        sim = orbc.Simulation(precision=orbc.MIXED, seed=7, skin=args.skin)
+        rng = np.random.default_rng(7)
+        kl, kp = e["kT"][e["lipid_group"]], e["kT"][e["protein_group"]]
+        lv = orbc.systems.maxwell(np.zeros(len(kl), np.int32), 1.0, rng) * np.sqrt(kl)[:, None]
+        pv = orbc.systems.maxwell(e["protein_type"], 1.0, rng) * np.sqrt(kp)[:, None]
+        sim.set_state(orbc.LIPID, e["lipid_x"], e["lipid_n"], e["lipid_id"], v=lv)
+        sim.set_state(orbc.PROTEIN, e["protein_x"], e["protein_n"], e["protein_id"], v=pv, type=e["protein_type"])
+        sim.set_bonds(e["bond_type"], e["bond_i"], e["bond_j"])
+        sim.set_groups(orbc.LIPID, e["lipid_group"], e["kT"])
+        sim.set_groups(orbc.PROTEIN, e["protein_group"], e["kT"])
+        n = len(e["lipid_x"]) + len(e["protein_x"])
+        desc = (f"batched ensemble: 64 spherical cells from geodesic meshes through the initialiser (frequency 1-9, "
+                f"counts log-uniform in 5k-300k), {n} particles, {len(e['bond_type'])} bonds, member kBT "
+                f"0.22 x U(0.9, 1.1), Langevin, one store and one fused launch per step; Voronoi cells "
+                f"updated every 2 steps, stray-lipid cleanup every 60")
+        steps = args.steps if args.steps != 1000 else 500
+    elif args.workload == "ensemble":
         e = orbc.vesicle_ensemble(n_members=64, seed=0)
         sim = orbc.Simulation(precision=orbc.MIXED, seed=7, skin=args.skin)
         rng = np.random.default_rng(7)
@@ -253,9 +271,9 @@ def side_workload(args):
                       type=np.zeros(0, np.int32))
         sim.set_groups(orbc.LIPID, e["group"], e["kT"])
         n = len(e["lipid_x"])
-        desc = (f"batched ensemble: 64 lipid vesicles, {n} particles (5k-300k each, log-uniform), member kBT "
-                f"0.22 x U(0.9, 1.1), Langevin, one store and one fused launch per step; Voronoi cells "
-                f"updated every 2 steps, stray-lipid cleanup every 60")
+        desc = (f"batched ensemble: 64 lipid vesicles from random points on spheres, {n} particles (5k-300k "
+                f"each, log-uniform), member kBT 0.22 x U(0.9, 1.1), Langevin, one store and one fused launch per "
+                f"step; Voronoi cells updated every 2 steps, stray-lipid cleanup every 60")
         steps = args.steps if args.steps != 1000 else 500
     else:
         sysd = orbc.planar_membrane(seed=1)
@@ -317,6 +335,9 @@ def main():
     ap.add_argument("--volume-strength", type=float, default=0.05,
                     help="constrain_volume strength (0: no volume constraint in the loop)")
     ap.add_argument("--lloyd", type=int, default=64, help="Lloyd iterations of the Voronoi init")
+    ap.add_argument("--vesicles", choices=["mesh", "random"], default="mesh",
+                    help="ensemble members: mesh = geodesic spheres through the initialiser (proteins, spectrin, "
+                         "bonds); random = lipid-only random points on spheres (init_random_sphere)")
     ap.add_argument("--workload", choices=["rbc", "ensemble", "planar"], default="rbc",
                     help="rbc: the full-size RBC, mixed and FP64 (the headline line); ensemble: 64 vesicles "
                          "of uneven size with their own kBT in one store (mixed, Langevin); planar: the "

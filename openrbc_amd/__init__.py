@@ -526,11 +526,11 @@ def compute_temperature(sim: Simulation) -> float:
 
 
 from .systems import (evans_fung, icosphere, init_rbc, load_example_large, planar_membrane, planar_patch,  # noqa: E402
-                      random_sphere, system_to_simulation, vesicle_ensemble)
+                      random_sphere, system_to_simulation, vesicle_ensemble, vesicle_ensemble_mesh)
 
 __all__ = [
     "Config", "Container", "Simulation", "OrbcError", "default_config", "load_library",
     "compute_pairwise_fused", "compute_bonded", "integrate", "constrain_volume", "compute_temperature",
     "read_frame", "POSITION", "ROTATION", "VORONOI", "VELOCITY", "FORCE",
-    "evans_fung", "icosphere", "init_rbc", "load_example_large", "planar_membrane", "planar_patch", "random_sphere", "system_to_simulation", "vesicle_ensemble",
+    "evans_fung", "icosphere", "init_rbc", "load_example_large", "planar_membrane", "planar_patch", "random_sphere", "system_to_simulation", "vesicle_ensemble", "vesicle_ensemble_mesh",
 ]

@@ -614,8 +614,12 @@ void rebuild(orbc_ctx *c) {
         c->halo_ok = true;
     }
     // 7c. skin budgets of the particles left without list entries (open boxes)
-    if (!any_per)
-        launch_budget(n, c->x[c->cur], c->type, c->nnbr, g, c->cstart, c->cend, (float)(0.5 * c->cfg.skin), c->bud2, s);
+    if (!any_per) {
+        // scratch as for the outer-shell pass (free again after the list build)
+        const FarScratch fs{c->keys_alt, c->vals_alt, c->vals, c->flags + 7, c->sort_temp};
+        launch_budget(n, c->x[c->cur], c->type, c->nnbr, g, c->cstart, c->cend, (float)(0.5 * c->cfg.skin), c->bud2,
+                      fs, s);
+    }
     // 8. bond lists
     if (c->nbonds > 0)
         launch_bondlist(n, c->stride, c->slot, c->inv, c->bstart, c->bpartner, c->btype_d, c->bnd, c->nbnd, s);
@@ -680,7 +684,8 @@ void ensure_list(orbc_ctx *c) {
     ensure_live(c);
     if (c->moved) {
         zero_flag(c, 0);
-        launch_disp(c->n, c->x[c->cur], c->anc, c->rb, c->ag, c->flags, c->stream);
+        const bool per = c->cfg.periodic[0] || c->cfg.periodic[1] || c->cfg.periodic[2];
+        launch_disp(c->n, c->x[c->cur], c->anc, c->rb, c->ag, per ? nullptr : c->bud2, c->nnbr, c->flags, c->stream);
         c->moved = false;
     }
     const unsigned bits = read_flag(c, 0);

@@ -580,9 +580,12 @@ __global__ __launch_bounds__(BLOCK) void k_bondlist(long long n, long long strid
     nbnd[i] = (uint8_t)(b1 - b0);
 }
 
-// max squared displacement since the last build, from the FP64 state
+// max squared displacement since the last build, from the FP64 state; a
+// particle with a skin budget (bud2, k_budget) counts 0 within it and
+// "infinite" past it, as in the skin check of the step kernels
 __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, const uint32_t *anc, const float *rb,
-                                                Anchor g, unsigned *disp_max) {
+                                                Anchor g, const float *bud2, const uint32_t *nnbr,
+                                                unsigned *disp_max) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     float d2 = 0.f;
     if (i < n) {
@@ -593,6 +596,7 @@ __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, co
             d2 += e * e;
         }
         if (!isfinite(d2)) d2 = 3.0e38f;
+        if (bud2 && nnbr[i] == 0 && bud2[i] > 0.f) d2 = d2 > bud2[i] ? 3.0e38f : 0.f;
     }
     for (int off = 32; off > 0; off >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, off, 64));
     // read before the atomic: only waves that raise the maximum write
@@ -714,54 +718,126 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
 // entries gets the usual budget skin/2 (its unlisted pairs start at rc + skin
 // or more).  A particle with no entries at all (a lipid ejected from a
 // vesicle, drifting away) gets b_i = min_j (r_ij - rc_ij) / 2 over the
-// particles j it could interact with in the cells within `R` rings, capped by
-// (R h - max_j rc_ij) / 2 for everything beyond them (h the cell edge): then
-// any two particles keep d_i + d_j <= r_ij - rc_ij as long as each stays within
-// its own budget, and one drifting lipid no longer triggers a rebuild every
-// step.  Stores b_i^2 (FP32, minus a rounding margin) where it exceeds
-// (skin/2)^2, else 0 ("skin/2").  Open boxes only (the caller skips periodic
-// ones).
-__global__ __launch_bounds__(BLOCK) void k_budget(long long n, const double *x, const uint8_t *type,
-                                                  const uint32_t *nnbr, Grid g, const uint32_t *cstart,
-                                                  const uint32_t *cend, float half_skin, float *bud2) {
+// particles j it could interact with in the cells within R rings, capped by
+// (R h - max_j rc_ij) / 2 for everything beyond them (h the shortest cell
+// edge; a particle outside the cube of R rings is at least R h away): then any
+// two particles keep d_i + d_j <= r_ij - rc_ij as long as each stays within
+// its own budget.  R is the type's ring count; when that finds nothing closer
+// than its cap and few particles are isolated, the cube is widened to
+// BUDGET_FAR_RINGS, so a lipid flying off at several cutoffs per ten steps
+// keeps the list for tens of steps instead of two.  Every isolated particle
+// of one build uses the same widening (it depends only on their count), which
+// keeps the pairwise argument above.  Stores b_i^2 (FP32, minus a rounding
+// margin) where it exceeds (skin/2)^2, else 0 ("skin/2").  Open boxes only
+// (the caller skips periodic ones).
+constexpr int BUDGET_FAR_RINGS = 12;
+constexpr long long BUDGET_FAR_MAX = 512;   // isolated particles for which the cube is widened
+constexpr unsigned BUDGET_GRID = 512;       // blocks of k_budget
+
+__global__ __launch_bounds__(BLOCK) void k_budget_flags(long long n, const uint8_t *type, const uint32_t *nnbr,
+                                                        uint32_t *flag, float *bud2) {
     const long long i = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const int ti = type[i];
-    const int R = c_budget_rings[ti];
-    float out = 0.f;
-    if (R > 0 && nnbr[i] == 0) {
-        const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-        double cmax = 0.0;
-        for (int tj = 0; tj < NT; ++tj) cmax = fmax(cmax, (double)c_pair_cut[ti * NT + tj]);
-        const double h = fmin(g.csd[0], fmin(g.csd[1], g.csd[2]));
-        double best = 0.5 * (R * h - cmax);
-        const int cx = cell_coord(xi, g.lo[0], g.csd[0], g.dim[0]);
-        const int cy = cell_coord(yi, g.lo[1], g.csd[1], g.dim[1]);
-        const int cz = cell_coord(zi, g.lo[2], g.csd[2], g.dim[2]);
-        for (int dz = -R; dz <= R; ++dz) {
-            int zz;
-            if (!cell_step(g, 2, cz, dz, zz)) continue;
-            for (int dy = -R; dy <= R; ++dy) {
-                int yy;
-                if (!cell_step(g, 1, cy, dy, yy)) continue;
-                for (int dx = -R; dx <= R; ++dx) {
-                    int xx;
-                    if (!cell_step(g, 0, cx, dx, xx)) continue;
-                    const long long cl = cell_lin(xx, yy, zz, g);
-                    for (uint32_t j = cstart[cl]; j < cend[cl]; ++j) {
-                        if (j == (uint32_t)i) continue;
-                        const double c = c_pair_cut[ti * NT + type[j]];
-                        if (!(c > 0.0)) continue;
-                        const double ex = xi - x[3 * j], ey = yi - x[3 * j + 1], ez = zi - x[3 * j + 2];
-                        best = fmin(best, 0.5 * (sqrt(ex * ex + ey * ey + ez * ez) - c));
-                    }
-                }
-            }
-        }
-        const float b = (float)best - 1e-4f;
-        if (b > half_skin) out = b * b;
+    flag[i] = (c_budget_rings[type[i]] > 0 && nnbr[i] == 0) ? 1u : 0u;
+    bud2[i] = 0.f;
+}
+
+struct BudgetSite {
+    double x, y, z, cmax;
+    int ti, cx, cy, cz;
+};
+
+__device__ inline BudgetSite budget_site(long long i, const double *x, const uint8_t *type, const Grid &g) {
+    BudgetSite b;
+    b.x = x[3 * i]; b.y = x[3 * i + 1]; b.z = x[3 * i + 2];
+    b.ti = type[i];
+    b.cmax = 0.0;
+    for (int tj = 0; tj < NT; ++tj) b.cmax = fmax(b.cmax, (double)c_pair_cut[b.ti * NT + tj]);
+    b.cx = cell_coord(b.x, g.lo[0], g.csd[0], g.dim[0]);
+    b.cy = cell_coord(b.y, g.lo[1], g.csd[1], g.dim[1]);
+    b.cz = cell_coord(b.z, g.lo[2], g.csd[2], g.dim[2]);
+    return b;
+}
+
+// min over the particles j != i of the cells with cube offset q (0 <= q <
+// (2R+1)^3, skipped when within `inner` rings) of (r_ij - rc_ij) / 2
+__device__ inline double budget_cell(long long i, const BudgetSite &b, long long q, int R, int inner,
+                                     const double *x, const uint8_t *type, const Grid &g, const uint32_t *cstart,
+                                     const uint32_t *cend, double best) {
+    const int side = 2 * R + 1;
+    const int dx = (int)(q % side) - R, dy = (int)((q / side) % side) - R, dz = (int)(q / ((long long)side * side)) - R;
+    if (max(abs(dx), max(abs(dy), abs(dz))) <= inner) return best;
+    int xx, yy, zz;
+    if (!cell_step(g, 0, b.cx, dx, xx) || !cell_step(g, 1, b.cy, dy, yy) || !cell_step(g, 2, b.cz, dz, zz)) return best;
+    const long long cl = cell_lin(xx, yy, zz, g);
+    for (uint32_t j = cstart[cl]; j < cend[cl]; ++j) {
+        if (j == (uint32_t)i) continue;
+        const double c = c_pair_cut[b.ti * NT + type[j]];
+        if (!(c > 0.0)) continue;
+        const double ex = b.x - x[3 * j], ey = b.y - x[3 * j + 1], ez = b.z - x[3 * j + 2];
+        best = fmin(best, 0.5 * (sqrt(ex * ex + ey * ey + ez * ez) - c));
     }
-    bud2[i] = out;
+    return best;
+}
+
+__device__ inline double block_min(double v, double *s_red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    if (__lane_id() == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = s_red[0];
+    for (int w = 1; w < BLOCK / 64; ++w) r = fmin(r, s_red[w]);
+    __syncthreads();
+    return r;
+}
+
+// list[0, *count): the isolated particles (k_budget_flags, compacted in index
+// order).  Few: one block per particle, the cube of its type's rings and, if
+// nothing there is closer than that cube's cap, the cube of BUDGET_FAR_RINGS.
+// Many: one thread per particle over its type's rings.
+__global__ __launch_bounds__(BLOCK) void k_budget(const double *x, const uint8_t *type, Grid g,
+                                                  const uint32_t *cstart, const uint32_t *cend, const uint32_t *list,
+                                                  const uint32_t *count, float half_skin, float *bud2) {
+    __shared__ double s_red[BLOCK / 64];
+    const long long m = *count;
+    const double h = fmin(g.csd[0], fmin(g.csd[1], g.csd[2]));
+    if (m > BUDGET_FAR_MAX) {
+        for (long long q = (long long)blockIdx.x * BLOCK + threadIdx.x; q < m; q += (long long)gridDim.x * BLOCK) {
+            const long long i = list[q];
+            const BudgetSite b = budget_site(i, x, type, g);
+            const int R = c_budget_rings[b.ti];
+            const long long side = 2 * R + 1;
+            double best = 0.5 * (R * h - b.cmax);
+            for (long long k = 0; k < side * side * side; ++k)
+                best = budget_cell(i, b, k, R, -1, x, type, g, cstart, cend, best);
+            const float r = (float)best - 1e-4f;
+            if (r > half_skin) bud2[i] = r * r;
+        }
+        return;
+    }
+    for (long long q = blockIdx.x; q < m; q += gridDim.x) {          // block-uniform
+        const long long i = list[q];
+        const BudgetSite b = budget_site(i, x, type, g);
+        const int R = c_budget_rings[b.ti];
+        long long side = 2 * R + 1;
+        double best = 1e300;
+        for (long long k = threadIdx.x; k < side * side * side; k += BLOCK)
+            best = budget_cell(i, b, k, R, -1, x, type, g, cstart, cend, best);
+        best = block_min(best, s_red);
+        double cap = 0.5 * (R * h - b.cmax);
+        if (best >= cap && BUDGET_FAR_RINGS > R) {
+            // nothing within R rings limits the budget: the cells between R and
+            // BUDGET_FAR_RINGS rings
+            side = 2 * BUDGET_FAR_RINGS + 1;
+            double far = 1e300;
+            for (long long k = threadIdx.x; k < side * side * side; k += BLOCK)
+                far = budget_cell(i, b, k, BUDGET_FAR_RINGS, R, x, type, g, cstart, cend, far);
+            best = fmin(best, block_min(far, s_red));
+            cap = 0.5 * (BUDGET_FAR_RINGS * h - b.cmax);
+        }
+        const float r = (float)fmin(best, cap) - 1e-4f;
+        if (threadIdx.x == 0 && r > half_skin) bud2[i] = r * r;
+    }
 }
 
 inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -769,10 +845,14 @@ inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); 
 }  // namespace
 
 void launch_budget(long long n, const double *x, const uint8_t *type, const uint32_t *nnbr, const Grid &g,
-                   const uint32_t *cstart, const uint32_t *cend, float half_skin, float *bud2, hipStream_t s) {
+                   const uint32_t *cstart, const uint32_t *cend, float half_skin, float *bud2, const FarScratch &fs,
+                   hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_budget, dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, type, nnbr, g, cstart, cend, half_skin,
-                       bud2);
+    hipLaunchKernelGGL(k_budget_flags, dim3(nblk(n)), dim3(BLOCK), 0, s, n, type, nnbr, fs.flag, bud2);
+    exclusive_scan_u32(fs.flag, fs.pos, n, fs.scan_temp, s);
+    hipLaunchKernelGGL(k_far_compact, dim3(nblk(n)), dim3(BLOCK), 0, s, n, fs.flag, fs.pos, fs.list, fs.count);
+    hipLaunchKernelGGL(k_budget, dim3(BUDGET_GRID), dim3(BLOCK), 0, s, x, type, g, cstart, cend, fs.list, fs.count,
+                       half_skin, bud2);
 }
 
 void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
@@ -781,10 +861,10 @@ void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint3
     hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, type, ext, cnt, nbr16);
 }
 
-void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g, unsigned *disp_max,
-                 hipStream_t s) {
+void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g, const float *bud2,
+                 const uint32_t *nnbr, unsigned *disp_max, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_disp, dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, anc, rb, g, disp_max);
+    hipLaunchKernelGGL(k_disp, dim3(nblk(n)), dim3(BLOCK), 0, s, n, x, anc, rb, g, bud2, nnbr, disp_max);
 }
 
 void set_list_cutoffs(const float *cut2_list, const int *rings, const float *pair_cut, const int *budget_rings,

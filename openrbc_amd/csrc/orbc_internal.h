@@ -302,10 +302,13 @@ void launch_nlist(const NlistArgs &a, const uint32_t *occ, long long nocc, bool 
 void set_list_cutoffs(const float *cut2_list, const int *rings, const float *pair_cut, const int *budget_rings,
                       hipStream_t s);
 // skin budget of the particles without list entries (k_build.hip, k_budget)
+// (fs: scratch as for the outer-shell pass of launch_nlist)
 void launch_budget(long long n, const double *x, const uint8_t *type, const uint32_t *nnbr, const Grid &g,
-                   const uint32_t *cstart, const uint32_t *cend, float half_skin, float *bud2, hipStream_t s);
-void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g,
-                 unsigned *disp_max, hipStream_t s);
+                   const uint32_t *cstart, const uint32_t *cend, float half_skin, float *bud2, const FarScratch &fs,
+                   hipStream_t s);
+// bud2: skin budgets of the last build (nullptr: skin/2 for every particle)
+void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g, const float *bud2,
+                 const uint32_t *nnbr, unsigned *disp_max, hipStream_t s);
 void launch_bondlist(long long n, long long stride, const uint32_t *slot, const uint32_t *inv,
                      const uint32_t *bstart, const uint32_t *bpartner, const uint8_t *btype,
                      uint32_t *bnd, uint8_t *nbnd, hipStream_t s);

@@ -220,6 +220,53 @@ def vesicle_ensemble(n_members: int = 64, n_min: int = 5000, n_max: int = 300000
                 group=np.concatenate(gs), kT=kts, radius=radius, centre=np.array(centres))
 
 
+def vesicle_ensemble_mesh(n_members: int = 64, n_min: int = 5000, n_max: int = 300000, rho: float = 1.05,
+                          kT: float = 0.22, spread: float = 0.1, gap: float = 12.0, seed: int = 0):
+    """Batched-ensemble workload on meshes: spherical cells from class-I
+    geodesic meshes (icosahedron faces split nu^2 ways) through the
+    initialiser (init_rbc rules, src/init_rbc.h:30-337: lipids, glycophorin,
+    actin, band-3, spectrin and their bonds).  init_rbc scales a mesh to a
+    mean edge of 80 nm, so a member of frequency nu holds about 3,540 nu^2
+    particles: nu is drawn so that the counts are log-uniform in [n_min,
+    n_max] (nu >= 1).  Member temperatures kT * U(1 - spread, 1 + spread);
+    members centred on a cubic lattice whose spacing leaves `gap` between
+    the largest ones; each member has its own placement seed.  Ids (and the
+    bond ids) are offset so they are unique across members.  Returns the
+    species arrays of init_rbc, lipid_group / protein_group (member of each
+    particle), kT, nu, radius and centre (per member)."""
+    rng = np.random.default_rng(seed)
+    target = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_members))
+    nus = np.maximum(1, np.rint(np.sqrt(target / 3540.0))).astype(np.int64)
+    kts = kT * rng.uniform(1 - spread, 1 + spread, n_members)
+    meshes = {nu: geodesic(int(nu)) for nu in np.unique(nus)}
+    members = [init_rbc(*meshes[nu], rho=rho, seed=int(seed) * 100003 + m + 1) for m, nu in enumerate(nus)]
+    radius = np.array([np.linalg.norm(s["lipid_x"], axis=1).max() for s in members])
+    side = int(np.ceil(n_members ** (1 / 3)))
+    pitch = 2 * radius.max() + gap
+    out = {k: [] for k in ("lipid_x", "lipid_n", "lipid_id", "protein_x", "protein_n", "protein_type", "protein_id",
+                           "bond_type", "bond_i", "bond_j", "lipid_group", "protein_group")}
+    centres = []
+    base = 0
+    for m, s in enumerate(members):
+        c = pitch * np.array([m % side, (m // side) % side, m // (side * side)], np.float64)
+        centres.append(c)
+        for sp in ("lipid", "protein"):
+            out[f"{sp}_x"].append(s[f"{sp}_x"] + c)
+            out[f"{sp}_n"].append(s[f"{sp}_n"])
+            out[f"{sp}_id"].append(s[f"{sp}_id"].astype(np.int64) + base)
+            out[f"{sp}_group"].append(np.full(len(s[f"{sp}_x"]), m, np.int32))
+        out["protein_type"].append(s["protein_type"])
+        out["bond_type"].append(s["bond_type"])
+        out["bond_i"].append(s["bond_i"].astype(np.int64) + base)
+        out["bond_j"].append(s["bond_j"].astype(np.int64) + base)
+        base += len(s["lipid_x"]) + len(s["protein_x"])
+    res = {k: np.concatenate(v) for k, v in out.items()}
+    for k in ("lipid_id", "protein_id", "bond_i", "bond_j"):
+        res[k] = res[k].astype(np.uint32)
+    res.update(kT=kts, nu=nus, radius=radius, centre=np.array(centres))
+    return res
+
+
 def planar_membrane(nx: int = 4096, ny: int = 4096, a: float = 1.0487, edge: float = 16.84, tilt: float = 0.02,
                     jitter: float = 0.02, layer: float = 2.25, seed: int = 0):
     """Large periodic planar membrane: an nx x ny hexagonal lipid lattice

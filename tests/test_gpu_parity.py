@@ -8,6 +8,7 @@ FP64 rounding.
 """
 import numpy as np
 import pytest
+from scipy.spatial import cKDTree
 
 import openrbc_amd as orbc
 from _systems import patch, periodic_patch
@@ -267,6 +268,50 @@ def test_drifting_lipids_keep_the_list_valid(precision, oracle):
         assert np.abs(got["x"] - st["x"]).max() < 1e-8
     # (mixed: the same budgets; its trajectory separates chaotically from the
     # FP64 oracle's, so only the rebuild count is checked)
+
+
+@pytest.mark.parametrize("precision", [orbc.FP64, orbc.MIXED])
+def test_far_lipids_get_wide_budgets(precision, oracle):
+    """A lipid far above the sheet (nothing within the type's two rings of
+    cells) gets its budget from the widened cube of cells: one falls from 17
+    above at 0.1 per step and lands on the sheet, one crosses high above it.
+    The FP64 run must follow the oracle stepping the exact pair set (a budget
+    too wide would miss the landing), and the two lipids may cost only a few
+    rebuilds."""
+    base = patch(10, 10, seed=4)
+    s = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+    c = base["x"][base["type"] == 0].mean(0)
+    extra_x = np.array([c + [0.0, 0.0, 17.0], c + [-20.0, 0.0, 25.0]])
+    extra_v = np.array([[0.0, 0.0, -10.0], [8.0, 0.0, 0.0]])
+    s["x"] = np.vstack([base["x"], extra_x])
+    s["n"] = np.vstack([base["n"], [[0, 0, 1.0], [0, 0, 1.0]]])
+    s["v"] = np.vstack([base["v"], extra_v])
+    s["o"] = np.vstack([base["o"], np.zeros((2, 3))])
+    s["type"] = np.concatenate([base["type"], [0, 0]]).astype(np.int32)
+    s["id"] = np.concatenate([base["id"], [base["id"].max() + 1, base["id"].max() + 2]]).astype(np.uint32)
+    seed = 5
+    nsteps = 190          # the falling lipid reaches the sheet at step ~160
+    runs = {}
+    for name, sysd in (("far", s), ("base", base)):
+        sim, lip, prot = make_sim(sysd, precision, seed=seed, skin=0.4)
+        sim.run(nsteps)
+        runs[name] = (gather(sim, lip, prot, "state"), sim.observables())
+    got, obs = runs["far"]
+    # widened cubes: the falling lipid halves its gap at each rebuild (17 -> 9.8
+    # -> 6.2 -> 4.4 -> 3.5 -> 3.1: 6 rebuilds), the crossing one needs one.
+    # Two rings only (budget <= 1.7) would cost about 18.
+    assert obs["rebuilds"] <= runs["base"][1]["rebuilds"] + 9, (obs["rebuilds"], runs["base"][1]["rebuilds"])
+    if precision == orbc.FP64:
+        st = {k: s[k].copy() for k in ("x", "v", "n", "o")}
+        st.update(type=s["type"], id=s["id"])
+        touched = False
+        for step in range(nsteps):
+            f, t, _, _ = oracle.forces(st["x"], st["n"], st["type"], bonds=s["bonds"])
+            touched = touched or np.abs(f[-2]).max() > 0
+            st["f"], st["t"] = f, t
+            oracle.langevin(st, 0.22, 0.01, 0.01, seed, step)
+        assert touched, "the falling lipid never reached the sheet"
+        assert np.abs(got["x"] - st["x"]).max() < 1e-8
 
 
 @pytest.mark.parametrize("precision", [orbc.FP64, orbc.MIXED])
@@ -887,6 +932,85 @@ def test_ensemble_equals_separate_runs(precision, tol):
         x1 = one.get_state(orbc.LIPID)["x"]
         xe_m = xe[g == m]
         assert np.abs(xe_m - x1).max() < tol * max(1.0, np.abs(x1).max()), m
+
+
+def _mesh_member_sim(e, members, precision, seed=13, **kw):
+    """Members `members` of a vesicle_ensemble_mesh system (both species,
+    bonds, per-member kBT) in one Simulation; velocities drawn for the whole
+    ensemble so a member alone gets the same ones."""
+    ls, ps = np.isin(e["lipid_group"], members), np.isin(e["protein_group"], members)
+    rng = np.random.default_rng(4)
+    lv = rng.normal(0, 1, e["lipid_x"].shape) * np.sqrt(e["kT"][e["lipid_group"]])[:, None]
+    pv = orbc.systems.maxwell(e["protein_type"], 1.0, rng) * np.sqrt(e["kT"][e["protein_group"]])[:, None]
+    sim = orbc.Simulation(precision=precision, seed=seed, **kw)
+    sim.set_state(orbc.LIPID, e["lipid_x"][ls], e["lipid_n"][ls], e["lipid_id"][ls], v=lv[ls])
+    sim.set_state(orbc.PROTEIN, e["protein_x"][ps], e["protein_n"][ps], e["protein_id"][ps], v=pv[ps],
+                  type=e["protein_type"][ps])
+    keep = np.isin(e["bond_i"], e["protein_id"][ps])
+    sim.set_bonds(e["bond_type"][keep], e["bond_i"][keep], e["bond_j"][keep])
+    return sim, ls, ps
+
+
+@pytest.mark.parametrize("precision,tol", [(orbc.FP64, 1e-9), (orbc.MIXED, 1e-5)])
+def test_mesh_ensemble_equals_separate_runs(precision, tol):
+    """Cells from geodesic meshes through the initialiser (lipids, proteins,
+    spectrin bonds) with different kBT in one store evolve as each cell alone
+    at its own kBT."""
+    e = orbc.vesicle_ensemble_mesh(n_members=3, n_min=3000, n_max=16000, gap=10.0, seed=2)
+    assert len(e["protein_x"]) > 0 and len(e["bond_type"]) > 0
+    nsteps = 25
+    ens, ls, ps = _mesh_member_sim(e, [0, 1, 2], precision)
+    ens.set_groups(orbc.LIPID, e["lipid_group"], e["kT"])
+    ens.set_groups(orbc.PROTEIN, e["protein_group"], e["kT"])
+    ens.run(nsteps)
+    le, pe = ens.get_state(orbc.LIPID)["x"], ens.get_state(orbc.PROTEIN)["x"]
+    for m in range(3):
+        one, ls1, ps1 = _mesh_member_sim(e, [m], precision, kBT=float(e["kT"][m]))
+        one.run(nsteps)
+        for xe, x1 in ((le[e["lipid_group"] == m], one.get_state(orbc.LIPID)["x"]),
+                       (pe[e["protein_group"] == m], one.get_state(orbc.PROTEIN)["x"])):
+            assert np.abs(xe - x1).max() < tol * max(1.0, np.abs(x1).max()), m
+
+
+def test_mesh_ensemble_full_size_properties():
+    """The batched-ensemble workload of the benchmark at full shape (64 cells
+    from geodesic meshes, 5k-300k particles each, ~4.5M particles, 0.8M
+    bonds, mixed precision): after minimisation and 500 Langevin steps every
+    state is finite, no pair joins two members, each member's temperature
+    follows its own kBT, lists last many steps (no ejected lipids force
+    rebuilds) and the stray cleanup removes few lipids."""
+    e = orbc.vesicle_ensemble_mesh(n_members=64, seed=0)
+    sim, ls, ps = _mesh_member_sim(e, list(range(64)), orbc.MIXED, skin=0.6)
+    sim.set_groups(orbc.LIPID, e["lipid_group"], e["kT"])
+    sim.set_groups(orbc.PROTEIN, e["protein_group"], e["kT"])
+    sim.minimize(100)
+    sim.voronoi_init(0, 4)
+    r0 = sim.observables()["rebuilds"]
+    sim.run(500)
+    o = sim.observables()
+    assert o["rebuilds"] - r0 < 60, o["rebuilds"] - r0
+    assert o["lipids_deleted"] < 0.001 * len(e["lipid_x"]), o["lipids_deleted"]
+    g = e["lipid_group"]
+    st = sim.get_state(orbc.LIPID)
+    assert np.isfinite(st["x"]).all() and np.isfinite(st["v"]).all()
+    assert np.isfinite(sim.get_state(orbc.PROTEIN)["x"]).all()
+    # member of each surviving lipid by the nearest cell centre (the cells are
+    # 10 apart; the cleanup closes up the slots of the removed lipids)
+    assert len(st["x"]) == len(g) - o["lipids_deleted"]
+    _, gs = cKDTree(e["centre"]).query(st["x"])
+    T = np.bincount(gs, weights=np.sum(st["v"] ** 2, 1), minlength=64) / (3 * np.bincount(gs, minlength=64))
+    ratio = T / e["kT"]
+    med = np.median(ratio)
+    assert 0.5 < med < 1.1, ratio
+    assert np.all(np.abs(ratio / med - 1) < 0.2), ratio
+    # member kBT spans 0.2-0.24: with one shared thermostat T would not track it
+    assert np.corrcoef(T, e["kT"])[0, 1] > 0.7, (T, e["kT"])
+    pg = e["protein_group"]
+    p = sim.pairs()
+    gid = np.zeros(int(max(e["lipid_id"].max(), e["protein_id"].max())) + 1, np.int32)
+    gid[e["lipid_id"]] = g
+    gid[e["protein_id"]] = pg
+    assert np.array_equal(gid[p[:, 0]], gid[p[:, 1]])
 
 
 def test_ensemble_rejects_nose_hoover():

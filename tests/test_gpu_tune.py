@@ -54,17 +54,22 @@ def test_autotune_restores_state_and_applies_a_candidate(tmp_path, precision):
     line = json.loads(table.read_text().strip().splitlines()[-1])
     assert len(line["trials"]) == res["trials"] and line["best"]["skin"] == tun["skin"]
     assert line["n_lipid"] == o0["n_lipid"] and line["precision"] == ("fp64" if precision == orbc.FP64 else "mixed")
-    # continue: same trajectory as a fresh context with the chosen knobs, from the same state
+    # continue: the same trajectory as a fresh context that did the same set-up
+    # and then took the chosen knobs (to FP64 rounding: the trials left the
+    # store in another order, so neighbour sums can be ordered differently)
     sim.run(15)
-    ref = _sim(s, precision, **{**kw, "skin": tun["skin"], "block_halo": tun["block_halo"]})
-    ref.set_tuning(**tun)
+    ref = _sim(s, precision, **kw)
     ref.minimize(10)
     ref.voronoi_init(0, 4)
     ref.run(6)
+    ref.set_tuning(**tun)
     ref.run(15)
     a, b = sim.get_state(orbc.LIPID), ref.get_state(orbc.LIPID)
-    tol = 1e-8 if precision == orbc.FP64 else 1e-5
+    # mixed: a rounding-level difference of an FP64 position can move the
+    # FP32 mirror of that particle by one FP32 ulp (6e-8 of an O(1) offset),
+    # which the next steps carry into the trajectory at that level
+    tol = 1e-8 if precision == orbc.FP64 else 1e-6
     for k in ("x", "n"):
-        assert np.abs(a[k] - b[k]).max() < tol, k
+        assert np.abs(a[k] - b[k]).max() < tol, (k, np.abs(a[k] - b[k]).max())
     sim.close()
     ref.close()
