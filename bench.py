@@ -304,7 +304,10 @@ def side_workload(args):
             "data": "synthetic", "config": {"workload": desc, "n_particles": n, "rebuilds": rebuilds,
                                             "temperature": obs["temperature"],
                                             "lipids_deleted": obs["lipids_deleted"],
-                                            "step_kernel_ms": tm["step_kernel_ms"] / max(1, tm["step_launches"]),
+                                            # per fused Langevin launch (HIP events); None for the
+                                            # Nose-Hoover loop, whose kernels are not timed one by one
+                                            "step_kernel_ms": (tm["step_kernel_ms"] / tm["step_launches"]
+                                                               if tm["step_launches"] else None),
                                             "block_halos": {"blocks": obs["halo_blocks"],
                                                             "overflow": obs["halo_overflow"]}},
             "tuning": tuning}
