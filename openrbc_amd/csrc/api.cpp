@@ -815,6 +815,7 @@ void voronoi_set_affiliation(orbc_ctx *c, const int *aff_lipid, const int *aff_p
     }
     ORBC_HIP_CHECK(hipMemcpyAsync(c->voro.aff, hv.data(), 4 * c->n, hipMemcpyHostToDevice, c->stream));
     c->voro.aff_valid = true;
+    if (c->voro.csize) launch_fill_u32(c->voro.csize + c->voro.ncell, 0u, 1, c->stream);   // counts no longer current
     sync(c);
 }
 

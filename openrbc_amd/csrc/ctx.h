@@ -170,6 +170,7 @@ struct orbc_ctx {
         Gate gate;                         // skip gate of the batched run (VoroArgs::gate)
         // stray-lipid cleanup scratch: member ranges per cell key, delete flags
         uint32_t *st_start = nullptr, *st_end = nullptr, *del = nullptr;
+        uint32_t *csize = nullptr;         // lipid count per cell + "current" word (VoroArgs::csize)
     } voro;
     double list_life = 0.0;       // steps a neighbour list lasts on average (batch sizing)
     // ensemble members (orbc_set_groups): per-particle member and per-member kBT

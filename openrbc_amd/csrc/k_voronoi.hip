@@ -368,11 +368,20 @@ __global__ __launch_bounds__(BLOCK) void k_cell_sums(VoroArgs a, unsigned long l
         }
     }
     __syncthreads();
+    // a cell whose every member is in this block's table has no other
+    // contributor: its sums are stored (the sums are exact integers, so the
+    // result is the same as the atomic path's)
+    const bool sized = a.csize && a.csize[a.ncell] != 0u;
     for (int t = threadIdx.x; t < SUM_SLOTS; t += BLOCK) {
         const int k = tkey[t];
-        if (k >= 0)
+        if (k < 0) continue;
+        if (sized && (unsigned long long)a.csize[k] == tval[t][3]) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) sums[4 * k + c] = tval[t][c];
+        } else {
 #pragma unroll
             for (int c = 0; c < 4; ++c) atomicAdd(&sums[4 * k + c], tval[t][c]);
+        }
     }
 }
 
@@ -435,7 +444,7 @@ __global__ void k_center_finish(const double *center_partial, long long ncp, lon
 // reading (see k_centroid_finish).  Volume partials go to vol_partial.
 __global__ __launch_bounds__(BLOCK) void k_volume_finish(unsigned long long *sums, const float *c, long long m,
                                                          const double *center, double *normals,
-                                                         double *vol_partial, Gate gt) {
+                                                         double *vol_partial, uint32_t *csize, Gate gt) {
     if (gate_closed(gt)) return;
     const double s_center[3] = {center[0], center[1], center[2]};
     const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
@@ -444,6 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_volume_finish(unsigned long long *sum
         unsigned long long q[4];
         for (int d = 0; d < 4; ++d) { q[d] = sums[4 * k + d]; sums[4 * k + d] = 0ull; }
         const long long cnt = (long long)q[3];
+        if (csize) csize[k] = (uint32_t)cnt;
         double nx = (double)(long long)q[0] / kNrmScale;
         double ny = (double)(long long)q[1] / kNrmScale;
         double nz = (double)(long long)q[2] / kNrmScale;
@@ -459,6 +469,7 @@ __global__ __launch_bounds__(BLOCK) void k_volume_finish(unsigned long long *sum
         }
         normals[3 * k] = nx; normals[3 * k + 1] = ny; normals[3 * k + 2] = nz;
     }
+    if (csize && blockIdx.x == 0 && threadIdx.x == 0) csize[m] = 1u;   // counts current (read by later kernels)
     block_sum_store(vol, vol_partial);
 }
 
@@ -624,7 +635,7 @@ void launch_volume(const VoroArgs &a, unsigned long long *sums, double *normals,
         hipLaunchKernelGGL(k_cell_sums<true>, dim3((unsigned)((a.n + SUM_CHUNK - 1) / SUM_CHUNK)), dim3(BLOCK), 0, s,
                            a, sums);
     hipLaunchKernelGGL(k_volume_finish, dim3((unsigned)nbc), dim3(BLOCK), 0, s, sums, a.centroids, a.ncell,
-                       partial + 4 * nbc, normals, partial + 3 * nbc, a.gate);
+                       partial + 4 * nbc, normals, partial + 3 * nbc, a.csize, a.gate);
     hipLaunchKernelGGL(k_volume_force, dim3(1), dim3(64), 0, s, partial + 3 * nbc, nbc, target, strength, scal,
                        a.gate);
 }

@@ -405,6 +405,11 @@ struct VoroArgs {
     // exact.  cdisp null: the index is current.
     float *cref;
     unsigned *cdisp;
+    // lipid count per cell of the current affiliation, written by
+    // k_volume_finish; csize[ncell] != 0 says they are current (a partition
+    // clears it).  A block of k_cell_sums holding every member of a cell
+    // stores that cell's sums instead of adding them atomically.  Null: none.
+    uint32_t *csize;
 };
 void launch_morton_ref(const float *pts, long long n, double bsize, uint32_t *keys, hipStream_t s);
 void launch_centroid_keys(const float *c, long long ncell, Grid g, uint32_t *keys, uint32_t *vals, hipStream_t s,

@@ -71,6 +71,7 @@ VoroArgs voro_args(orbc_ctx *c) {
     a.cref = c->voro.cref;
     a.cdisp = c->voro.cdisp;
     a.cg_dev = c->cg_dev;
+    a.csize = c->voro.csize;
     return a;
 }
 
@@ -101,6 +102,8 @@ void partition_all(orbc_ctx *c) {
     VoroArgs a = voro_args(c);
     launch_nearest(a, nullptr, c->n, c->x[c->cur], c->voro.aff, c->voro.aff_valid, c->stream);
     c->voro.aff_valid = true;
+    // the cell counts of the old affiliation are no longer current
+    if (c->voro.csize) launch_fill_u32(c->voro.csize + c->voro.ncell, 0u, 1, c->stream, a.gate);
 }
 
 void update_centroids(orbc_ctx *c) {
@@ -126,6 +129,8 @@ void voro_alloc(orbc_ctx *c, long long ncell) {
     V.normals = c->mem.alloc<double>(3 * ncell);
     V.cref = c->mem.alloc<float>(3 * ncell);
     V.cdisp = c->mem.alloc<unsigned>(1);
+    V.csize = c->mem.alloc<uint32_t>(ncell + 1);
+    launch_fill_u32(V.csize + ncell, 0u, 1, c->stream);
     // centre partials (3 per block), volume partials (1 per block), the centre (3)
     V.partial = c->mem.alloc<double>(4 * (ncell / BLOCK + 2) + 16);
     V.ck = c->mem.alloc<uint32_t>(ncell); V.cv = c->mem.alloc<uint32_t>(ncell);
@@ -142,7 +147,7 @@ namespace orbc {
 void voronoi_free(orbc_ctx *c) {
     auto &V = c->voro;
     void *ps[] = {V.aff, V.aff_alt, V.c, V.sums, V.normals, V.partial, V.ck, V.cv, V.ck2, V.cv2, V.cg_start, V.cg_end,
-                  V.temp, V.cref, V.cdisp, V.st_start, V.st_end, V.del};
+                  V.temp, V.cref, V.cdisp, V.st_start, V.st_end, V.del, V.csize};
     for (void *p : ps) c->mem.release(p);
     V = orbc_ctx::Voro{};
 }

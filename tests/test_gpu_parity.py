@@ -451,6 +451,37 @@ def c1():
                 bonds=(s["bond_type"], bi, bj))
 
 
+def test_c1_cell_sums_stored_and_atomic_agree(c1, oracle):
+    """Per-cell sums over 110 blocks of 2048 particles: the first volume
+    evaluation after a partition adds every block's part atomically and
+    records the cell counts; later ones store the sums of the cells a block
+    holds whole.  Both must give the oracle's volume and push, and the
+    centroid update that follows (stored sums too) the member means."""
+    s = c1
+    sim, lip, prot, c = _voro_sim(s)
+    sim.voronoi_set_centroids(c)
+    aff = _aff(sim, lip, prot, len(s["type"]))
+    fo = np.zeros_like(s["x"])
+    Vo = oracle.constrain_volume(s["n"], s["type"], aff, c, 3.0, 0.05, fo)
+    sim.compute_pairwise(zero_first=True)
+    f_prev, _ = gather(sim, lip, prot)
+    for call in range(3):            # 0: atomic after the partition; 1, 2: counts current
+        V = sim.constrain_volume(3.0, 0.05)
+        f, _ = gather(sim, lip, prot)
+        assert abs(V - Vo) < 1e-10 * max(1.0, abs(Vo)), call
+        assert np.abs((f - f_prev) - fo).max() < 1e-10 * max(1.0, np.abs(fo).max()), call
+        f_prev = f
+    sim.voronoi_update()
+    c2, _, _ = sim.voronoi_get()
+    member = np.where(s["type"] == 0, aff, -1)
+    cnt = np.bincount(member[member >= 0], minlength=len(c))
+    exp = c.astype(np.float64).copy()
+    for d in range(3):
+        sx = np.bincount(member[member >= 0], weights=s["x"][member >= 0, d], minlength=len(c))
+        exp[cnt > 0, d] = sx[cnt > 0] / cnt[cnt > 0]
+    assert np.abs(c2 - exp).max() < 2e-6 * max(1.0, np.abs(exp).max())
+
+
 @pytest.mark.parametrize("precision,tol", [(orbc.FP64, EPS_FP64), (orbc.MIXED, 5e-5)])
 def test_c1_forces_vs_oracle(c1, precision, tol, oracle):
     """C1 as the initialiser leaves it: lipids placed at random 0.55 apart, so
