@@ -469,6 +469,7 @@ def main():
             "precision": hname, "mesh": args.mesh,
             "block_halos": {"blocks": head["halo_blocks"], "overflow": head["halo_overflow"]},
             "graphs": {"replays_in_timed_region": head["graph_replays"], "captured": head["graph_captures"]},
+            "rebuilds_in_timed_region": head["rebuilds"],
         },
         "roofline": {
             "bound": "hbm",
