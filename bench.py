@@ -425,7 +425,8 @@ def main():
                          energy_per_particle=dict(start=e0 / n, end=e1 / n, drift=(e1 - e0) / n),
                          halo_blocks=obs["halo_blocks"], halo_overflow=obs["halo_overflow"],
                          graph_replays=obs["graph_replays"] - obs0["graph_replays"],
-                         graph_captures=obs["graph_captures"])
+                         graph_captures=obs["graph_captures"],
+                         graph_captures_timed=obs["graph_captures"] - obs0["graph_captures"])
         print(f"# {name}: {res[name]['value']:.4g} particle-steps/s, {res[name]['ms_per_step']:.3f} ms/step, "
               f"kernel {kern_ms:.3f} ms, rebuilds {rebuilds} ({tm['rebuild_ms']:.1f} ms), "
               f"T={obs['temperature']:.4f}, E/N {e0 / n:.5f} -> {e1 / n:.5f}, "
@@ -468,7 +469,8 @@ def main():
             "n_particles": n, "n_bonds": nb, "pairs_within_cutoff": e_pairs, "skin": args.skin,
             "precision": hname, "mesh": args.mesh,
             "block_halos": {"blocks": head["halo_blocks"], "overflow": head["halo_overflow"]},
-            "graphs": {"replays_in_timed_region": head["graph_replays"], "captured": head["graph_captures"]},
+            "graphs": {"replays_in_timed_region": head["graph_replays"], "captured": head["graph_captures"],
+                       "captured_in_timed_region": head["graph_captures_timed"]},
             "rebuilds_in_timed_region": head["rebuilds"],
         },
         "roofline": {
