@@ -92,6 +92,7 @@ struct orbc_ctx {
     int kmax = 0;
     // block halos of the mixed step (launch_halo), valid after a rebuild when halo_ok
     uint32_t *halo_ext = nullptr, *halo_cnt = nullptr, *nbr16 = nullptr;
+    uint8_t *hord = nullptr;      // lane order per halo block (halo_nblk_cap * BLOCK)
     long long halo_nblk_cap = 0, nbr16_cap = 0;
     bool halo_ok = false;
     uint32_t *bnd = nullptr;

@@ -628,9 +628,10 @@ __device__ inline uint32_t halo_hash(uint32_t j) { return (j * 2654435761u) >> (
 
 __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, const uint32_t *nbr,
                                                 const uint32_t *nnbr, const uint8_t *type, uint32_t *ext,
-                                                uint32_t *ext_cnt, uint32_t *nbr16) {
+                                                uint32_t *ext_cnt, uint32_t *nbr16, uint8_t *hord) {
     __shared__ uint32_t s_key[HALO_HASH];
     __shared__ uint16_t s_val[HALO_HASH];
+    __shared__ int s_nn[BLOCK];
     __shared__ int s_cnt, s_over;
     const int t = threadIdx.x;
     const long long b = blockIdx.x, i0 = b * BLOCK, i = i0 + t;
@@ -676,7 +677,21 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
         return;
     }
     if (t == 0) ext_cnt[b] = (uint32_t)s_cnt;
+    // lane order of the step kernels: the block's particles by list length,
+    // longest first (ties by position; positions past n last), so that each
+    // wave walks rows of similar length.  Lane `r` of the block takes particle
+    // i0 + hord[i0 + r], whose row is written to column i0 + r.
+    s_nn[t] = i < n ? nn : -1;
+    __syncthreads();
+    const int mine = s_nn[t];
+    int rank = 0;
+    for (int u = 0; u < BLOCK; ++u) {
+        const int v = s_nn[u];
+        rank += (v > mine) || (v == mine && u < t);
+    }
+    hord[i0 + rank] = (uint8_t)t;
     if (i >= n) return;
+    const long long col = i0 + rank;        // < n: active particles rank first
     // local indices, HB entries (HB / 2 words) per batch; the row's words
     // [0, halo_row_words(nn)) are written, entries past nn being the particle
     // itself, and nn <= kmax keeps them inside halo_words(kmax)
@@ -708,7 +723,7 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
                 }
                 word |= (l | (tb[2 * q + hh] << 10)) << (16 * hh);
             }
-            nbr16[(long long)(m0 + q) * stride + i] = word;
+            nbr16[(long long)(m0 + q) * stride + col] = word;
         }
     }
 }
@@ -856,9 +871,9 @@ void launch_budget(long long n, const double *x, const uint8_t *type, const uint
 }
 
 void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
-                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, hipStream_t s) {
+                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, uint8_t *hord, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, type, ext, cnt, nbr16);
+    hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, type, ext, cnt, nbr16, hord);
 }
 
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g, const float *bud2,

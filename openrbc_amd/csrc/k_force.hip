@@ -777,16 +777,17 @@ __device__ __forceinline__ PQ halo_stage(const StepArgs &a, long long blk, bool 
     return qi;
 }
 
-// pair and bond forces of active particle i from the staged halo (or, for an
-// overflowed block, gathered one neighbour at a time)
-__device__ __forceinline__ void halo_accumulate(const StepArgs &a, const Anchor &ag, long long i, const PQ &qi, uint32_t hcnt,
-                                                const HaloLDS &L, Acc<double> &acc) {
+// pair and bond forces of active particle i from the staged halo (its row at
+// column `col` of the 16-bit list), or, for an overflowed block, gathered one
+// neighbour at a time
+__device__ __forceinline__ void halo_accumulate(const StepArgs &a, const Anchor &ag, long long i, long long col,
+                                                const PQ &qi, uint32_t hcnt, const HaloLDS &L, Acc<double> &acc) {
     const int ti = __float_as_int(qi.n.w);
     const AnchorI ai = unpack(__float_as_uint(qi.x.w), ag);
     const int nn = (int)a.nnbr[i];
     const int crow = ti * NT;
     if (hcnt != HALO_OVERFLOW) {
-        range_halo(nn, a.stride, a.nbr16 + i, L.x, L.n, L.pp, crow, ag, ai, qi.x, qi.n, acc);
+        range_halo(nn, a.stride, a.nbr16 + col, L.x, L.n, L.pp, crow, ag, ai, qi.x, qi.n, acc);
     } else {
         for (int k = 0; k < nn; ++k) {
             const PQ q = a.pq_in[a.nbr[(long long)k * a.stride + i]];
@@ -829,15 +830,16 @@ __device__ __forceinline__ void halo_stage64(const StepArgs &a, long long blk, b
     __syncthreads();
 }
 
-__device__ __forceinline__ void halo_accumulate64(const StepArgs &a, const Anchor &ag, long long i, uint32_t hcnt,
-                                                  const HaloLDS64 &L, const FFS<double> &t, Acc<double> &acc) {
+__device__ __forceinline__ void halo_accumulate64(const StepArgs &a, const Anchor &ag, long long i, long long col,
+                                                  uint32_t hcnt, const HaloLDS64 &L, const FFS<double> &t,
+                                                  Acc<double> &acc) {
     const double xi = a.x_in[3 * i], yi = a.x_in[3 * i + 1], zi = a.x_in[3 * i + 2];
     const double nix = a.n_in[3 * i], niy = a.n_in[3 * i + 1], niz = a.n_in[3 * i + 2];
     const int ti = a.type[i];
     const bool periodic = ag.L[0] > 0 || ag.L[1] > 0 || ag.L[2] > 0;
     const int nn = (int)a.nnbr[i];
     if (hcnt != HALO_OVERFLOW) {
-        const uint32_t *w16 = a.nbr16 + i;
+        const uint32_t *w16 = a.nbr16 + col;
         const int nw = halo_row_words(nn);
         for (int m = 0; m < nw; ++m) {
             const uint32_t w = w16[(long long)m * a.stride];
@@ -886,16 +888,28 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
     __shared__ HaloLDS hl;
     __shared__ HaloLDS64 hl64;
     const long long blk = block_index(a.swizzle);
-    const long long i = blk * BLOCK + threadIdx.x;
-    const bool act = i < a.n;
+    // this lane's position in the block (its column of the 16-bit list),
+    // recomputed where needed rather than kept live
+#define ORBC_LANE_COL (blk * BLOCK + (long long)threadIdx.x)
     uint32_t hcnt = HALO_OVERFLOW;
     PQ qi{};
     if constexpr (HALO && MIXED) {
-        qi = halo_stage(a, blk, act, i, hl, hcnt);
+        qi = halo_stage(a, blk, ORBC_LANE_COL < a.n, ORBC_LANE_COL, hl, hcnt);
     } else {
         load_ffs(&tab);
-        if constexpr (HALO) halo_stage64(a, blk, act, i, hl64, hcnt);
+        if constexpr (HALO) halo_stage64(a, blk, ORBC_LANE_COL < a.n, ORBC_LANE_COL, hl64, hcnt);
     }
+    // the particle of this lane: the block's lane order (k_halo) when its
+    // neighbours are staged, else the lane's own position
+    long long i = ORBC_LANE_COL;
+    if constexpr (HALO) {
+        if (hcnt != HALO_OVERFLOW) {
+            const int l = a.hord[ORBC_LANE_COL];
+            i = blk * BLOCK + l;
+            if constexpr (MIXED) { qi.x = hl.x[l]; qi.n = hl.n[l]; }
+        }
+    }
+    const bool act = i < a.n;
     float d2 = 0.f;
     bool bad = false;
     if (act) {
@@ -906,9 +920,9 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
         if constexpr (HALO && MIXED) {
             ti = __float_as_int(qi.n.w);
             wi = __float_as_uint(qi.x.w);
-            halo_accumulate(a, ag, i, qi, hcnt, hl, acc);
+            halo_accumulate(a, ag, i, ORBC_LANE_COL, qi, hcnt, hl, acc);
         } else if constexpr (HALO) {
-            halo_accumulate64(a, ag, i, hcnt, hl64, tab, acc);
+            halo_accumulate64(a, ag, i, ORBC_LANE_COL, hcnt, hl64, tab, acc);
             ti = a.type[i];
             wi = a.anc[i];
         } else if constexpr (MIXED) {
@@ -1020,26 +1034,34 @@ __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, fl
     __shared__ HaloLDS hl;
     __shared__ HaloLDS64 hl64;
     const long long blk = block_index(a.swizzle);
-    const long long i = blk * BLOCK + threadIdx.x;
-    const bool act = i < a.n;
     uint32_t hcnt = HALO_OVERFLOW;
     PQ qh{};
     if constexpr (HALO && MIXED) {
-        qh = halo_stage(a, blk, act, i, hl, hcnt);
+        qh = halo_stage(a, blk, ORBC_LANE_COL < a.n, ORBC_LANE_COL, hl, hcnt);
     } else {
         load_ffs(&tab);
-        if constexpr (HALO) halo_stage64(a, blk, act, i, hl64, hcnt);
+        if constexpr (HALO) halo_stage64(a, blk, ORBC_LANE_COL < a.n, ORBC_LANE_COL, hl64, hcnt);
     }
+    // the particle of this lane (see step_langevin)
+    long long i = ORBC_LANE_COL;
+    if constexpr (HALO) {
+        if (hcnt != HALO_OVERFLOW) {
+            const int l = a.hord[ORBC_LANE_COL];
+            i = blk * BLOCK + l;
+            if constexpr (MIXED) { qh.x = hl.x[l]; qh.n = hl.n[l]; }
+        }
+    }
+    const bool act = i < a.n;
     double ke = 0.0;
     if (act) {
         double f[3], t[3];
         int ti;
         Acc<double> acc;
         if constexpr (HALO && MIXED) {
-            halo_accumulate(a, a.anc_grid, i, qh, hcnt, hl, acc);
+            halo_accumulate(a, a.anc_grid, i, ORBC_LANE_COL, qh, hcnt, hl, acc);
             ti = __float_as_int(qh.n.w);
         } else if constexpr (HALO) {
-            halo_accumulate64(a, a.anc_grid, i, hcnt, hl64, tab, acc);
+            halo_accumulate64(a, a.anc_grid, i, ORBC_LANE_COL, hcnt, hl64, tab, acc);
             ti = a.type[i];
         } else if constexpr (MIXED) {
             const PQ qi = a.pq_in[i];
@@ -1076,6 +1098,15 @@ __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, fl
             for (int d = 0; d < 3; ++d) { f64[3 * i + d] = f[d]; t64[3 * i + d] = q[d]; }
         }
         ke = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    if constexpr (HALO) {
+        // back to particle order, the order of the unfused reduction
+        if (hcnt != HALO_OVERFLOW) {          // block-uniform
+            __shared__ double s_ke[BLOCK];
+            s_ke[i - blk * BLOCK] = ke;
+            __syncthreads();
+            ke = s_ke[threadIdx.x];
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ke += __shfl_xor(ke, off, 64);

@@ -174,6 +174,7 @@ struct StepArgs {
     const uint32_t *nbr, *nnbr;
     // block halos (mixed path; null: every block gathers from nbr)
     const uint32_t *halo_ext, *halo_cnt, *nbr16;
+    const uint8_t *hord;     // lane order of each halo block (k_halo): lane r takes particle blk * 256 + hord[.. + r]
     const uint32_t *bnd;
     const uint8_t *nbnd;
     int has_bonds;
@@ -315,7 +316,7 @@ void launch_bondlist(long long n, long long stride, const uint32_t *slot, const 
 // block halos from the 32-bit list: ext (HALO_EXT per block), cnt (per block),
 // nbr16 (halo_words(kmax) words per particle, column stride `stride`)
 void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
-                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, hipStream_t s);
+                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, uint8_t *hord, hipStream_t s);
 
 // k_force.hip
 void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool bond, hipStream_t s);
