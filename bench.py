@@ -235,7 +235,10 @@ def traffic_profile(name: str, mesh: str):
             h.update(fh.read())
     if not prof.get("sources") or h.hexdigest() != prof.get("sources_sha256"):
         return None, os.path.relpath(tpath, ROOT) + " (measured on other kernel sources: not used)"
-    return prof.get("hbm_bytes_per_launch"), os.path.relpath(tpath, ROOT)
+    src = os.path.relpath(tpath, ROOT)
+    if prof.get("measured_with"):
+        src += f" (measured with {prof['measured_with']})"
+    return prof.get("hbm_bytes_per_launch"), src
 
 
 def side_workload(args):

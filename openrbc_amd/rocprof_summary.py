@@ -137,6 +137,7 @@ def traffic(args):
         "calibration": f"a {args.cal_mib} MiB device-to-device copy reads back as FETCH_SIZE {cf:.0f} KiB and "
                        f"WRITE_SIZE {cw:.0f} KiB",
         "commands": args.command, "sources": TRAFFIC_SOURCES, "sources_sha256": sources_sha256(TRAFFIC_SOURCES),
+        "measured_with": args.measured_with,
     }
     with open(args.out, "w") as fh:
         json.dump(prof, fh, indent=1)
@@ -152,7 +153,8 @@ def traffic(args):
             f"* calibration: {prof['calibration']}, so reads are scaled by {fs:.3f} and writes by {ws:.3f}",
             f"* HBM traffic per launch: {hbm / 1e9:.3f} GB (reads {f_kib * fs * 1024 / 1e9:.3f} GB, "
             f"writes {w_kib * ws * 1024 / 1e9:.3f} GB)",
-            f"* kernel sources: {', '.join(TRAFFIC_SOURCES)} (sha256 {prof['sources_sha256'][:16]}...)", ""]))
+            f"* kernel sources: {', '.join(TRAFFIC_SOURCES)} (sha256 {prof['sources_sha256'][:16]}...)",
+            f"* run knobs: {args.measured_with}" if args.measured_with else "", ""]))
     print(json.dumps(prof))
 
 
@@ -170,6 +172,8 @@ def main(argv=None):
     f.add_argument("--cal-mib", type=int, required=True)
     f.add_argument("--last", type=int, required=True)
     f.add_argument("--command", action="append", required=True)
+    f.add_argument("--measured-with", dest="measured_with", default="",
+                   help="the run's knobs (skin, block halos, ...), recorded beside the figure")
     a = ap.parse_args(argv)
     (trace if a.what == "trace" else traffic)(a)
 
