@@ -427,6 +427,7 @@ def main():
                          temperature=obs["temperature"], potential_per_particle=obs["potential"] / n,
                          energy_per_particle=dict(start=e0 / n, end=e1 / n, drift=(e1 - e0) / n),
                          halo_blocks=obs["halo_blocks"], halo_overflow=obs["halo_overflow"],
+                         halo_capacity=sim.tuning()["halo_capacity"],
                          graph_replays=obs["graph_replays"] - obs0["graph_replays"],
                          graph_captures=obs["graph_captures"],
                          graph_captures_timed=obs["graph_captures"] - obs0["graph_captures"])
@@ -450,6 +451,9 @@ def main():
     head = res.get("mixed") or res.get("fp64")
     hname = "mixed" if "mixed" in res else "fp64"
     traffic, traffic_src = traffic_profile(hname, args.mesh)
+    if traffic is not None and (not head["halo_blocks"] or head["halo_capacity"] != 896):
+        # the profile is of the default halo kernel (896 local indices)
+        traffic, traffic_src = None, traffic_src + " (of the 896-index halo kernel; this run used another: not used)"
     line = {
         "metric": METRIC,
         "value": head["value"],
@@ -471,7 +475,8 @@ def main():
                         f"volume (strength {args.volume_strength})",
             "n_particles": n, "n_bonds": nb, "pairs_within_cutoff": e_pairs, "skin": args.skin,
             "precision": hname, "mesh": args.mesh,
-            "block_halos": {"blocks": head["halo_blocks"], "overflow": head["halo_overflow"]},
+            "block_halos": {"blocks": head["halo_blocks"], "overflow": head["halo_overflow"],
+                            "capacity": head["halo_capacity"]},
             "graphs": {"replays_in_timed_region": head["graph_replays"], "captured": head["graph_captures"],
                        "captured_in_timed_region": head["graph_captures_timed"]},
             "rebuilds_in_timed_region": head["rebuilds"],
@@ -484,8 +489,8 @@ def main():
             "frac": head["achieved_GBs"] * 1e9 / HBM_PEAK,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": ("k_step_langevin_halo" if head["halo_blocks"] else "k_step_langevin")
-                      + " (fused pair + bond + Langevin)",
+            "kernel": (("k_step_langevin_halo" + ("_wide" if head["halo_capacity"] == 1024 else ""))
+                       if head["halo_blocks"] else "k_step_langevin") + " (fused pair + bond + Langevin)",
             "kernel_ms": head["kernel_ms"],
             "kernel_timing": f"HIP events around each fused kernel on the context stream, {args.steps} steps "
                              f"launched one by one right after the timed region",

@@ -292,6 +292,8 @@ typedef struct orbc_tuning {
     double ms_per_step;          /* measured, chosen configuration (orbc_autotune) */
     double ms_per_step_start;    /* measured, configuration before tuning (orbc_autotune) */
     int trials;
+    int halo_capacity;           /* local indices per block halo: 896 (5 blocks per CU) or 1024 (4 per CU,
+                                    fewer overflowing blocks); 0 in orbc_set_tuning: unchanged */
 } orbc_tuning;
 int orbc_autotune(orbc_ctx *ctx, int warm_steps, int trial_steps, const char *table_path, orbc_tuning *out);
 /* apply a configuration (e.g. a line of a stored table); takes effect at the next rebuild */

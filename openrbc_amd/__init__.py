@@ -70,6 +70,7 @@ class Tuning(C.Structure):
     _fields_ = [
         ("skin", C.c_double), ("block_halo", C.c_int), ("centroid_cell", C.c_double), ("batch", C.c_int),
         ("ms_per_step", C.c_double), ("ms_per_step_start", C.c_double), ("trials", C.c_int),
+        ("halo_capacity", C.c_int),
     ]
 
 
@@ -405,14 +406,15 @@ class Simulation:
                                          table.encode() if table else None, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
-    def set_tuning(self, skin: float, block_halo: int, centroid_cell: float, batch: int):
-        t = Tuning(skin=skin, block_halo=block_halo, centroid_cell=centroid_cell, batch=batch)
+    def set_tuning(self, skin: float, block_halo: int, centroid_cell: float, batch: int, halo_capacity: int = 0):
+        t = Tuning(skin=skin, block_halo=block_halo, centroid_cell=centroid_cell, batch=batch,
+                   halo_capacity=halo_capacity)
         self._check(self.L.orbc_set_tuning(self.h, C.byref(t)))
 
     def tuning(self) -> dict:
         t = Tuning()
         self._check(self.L.orbc_get_tuning(self.h, C.byref(t)))
-        return {k: getattr(t, k) for k in ("skin", "block_halo", "centroid_cell", "batch")}
+        return {k: getattr(t, k) for k in ("skin", "block_halo", "centroid_cell", "batch", "halo_capacity")}
 
     def synchronize(self):
         self._check(self.L.orbc_synchronize(self.h))

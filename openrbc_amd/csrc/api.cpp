@@ -603,7 +603,7 @@ void rebuild(orbc_ctx *c) {
         if (nb > c->halo_nblk_cap) {
             c->mem.release(c->halo_ext); c->mem.release(c->halo_cnt); c->mem.release(c->hord);
             c->halo_nblk_cap = std::max(nb, (c->stride + BLOCK - 1) / BLOCK);
-            c->halo_ext = c->mem.alloc<uint32_t>((size_t)c->halo_nblk_cap * HALO_EXT);
+            c->halo_ext = c->mem.alloc<uint32_t>((size_t)c->halo_nblk_cap * HALO_EXT_MAX);
             c->halo_cnt = c->mem.alloc<uint32_t>((size_t)c->halo_nblk_cap);
             c->hord = c->mem.alloc<uint8_t>((size_t)c->halo_nblk_cap * BLOCK);
         }
@@ -612,7 +612,8 @@ void rebuild(orbc_ctx *c) {
             c->nbr16_cap = words;
             c->nbr16 = c->mem.alloc<uint32_t>((size_t)words);
         }
-        launch_halo(n, c->stride, c->nbr, c->nnbr, c->type, c->halo_ext, c->halo_cnt, c->nbr16, c->hord, s);
+        launch_halo(n, c->stride, c->nbr, c->nnbr, c->type, c->halo_ext, c->halo_cnt, c->nbr16, c->hord, c->halo_cap, s);
+        c->halo_cap_built = c->halo_cap;
         c->halo_ok = true;
     }
     // 7c. skin budgets of the particles left without list entries (open boxes)
@@ -1090,7 +1091,10 @@ static StepArgs step_args(orbc_ctx *c) {
     a.pq_out = c->pq[out];
     a.type = c->type; a.id = c->id; a.anc = c->anc; a.rb = c->rb;
     a.nbr = c->nbr; a.nnbr = c->nnbr;
-    if (c->halo_ok) { a.halo_ext = c->halo_ext; a.halo_cnt = c->halo_cnt; a.nbr16 = c->nbr16; a.hord = c->hord; }
+    if (c->halo_ok) {
+        a.halo_ext = c->halo_ext; a.halo_cnt = c->halo_cnt; a.nbr16 = c->nbr16; a.hord = c->hord;
+        a.halo_cap = c->halo_cap_built;
+    }
     a.bnd = c->nbonds ? c->bnd : nullptr;
     a.nbnd = c->nbonds ? c->nbnd : nullptr;
     a.has_bonds = c->nbonds > 0;

@@ -93,6 +93,8 @@ struct orbc_ctx {
     // block halos of the mixed step (launch_halo), valid after a rebuild when halo_ok
     uint32_t *halo_ext = nullptr, *halo_cnt = nullptr, *nbr16 = nullptr;
     uint8_t *hord = nullptr;      // lane order per halo block (halo_nblk_cap * BLOCK)
+    int halo_cap = orbc::HALO_CAP;      // local indices of the next halo build (autotuner knob: HALO_CAP, HALO_CAP_WIDE)
+    int halo_cap_built = orbc::HALO_CAP;   // ... of the halos built at the last rebuild
     long long halo_nblk_cap = 0, nbr16_cap = 0;
     bool halo_ok = false;
     uint32_t *bnd = nullptr;

@@ -607,28 +607,28 @@ __global__ __launch_bounds__(BLOCK) void k_disp(long long n, const double *x, co
 // ---- block halos ------------------------------------------------------------
 // One workgroup per particle block b (particles i0 = b * BLOCK ...): the
 // distinct list entries outside the block go into an LDS hash set, and the
-// order in which they first arrive numbers them: ext[b * HALO_EXT + e] is
+// order in which they first arrive numbers them: ext[b * HALO_EXT_MAX + e] is
 // entry e.  Then each particle's entries are rewritten as 16-bit entries: the
 // local index (t < BLOCK: particle i0 + t; BLOCK + e: ext entry e) in bits
-// 0-9 and the neighbour's type in bits 10-12, two per word, word m of
-// particle i at nbr16[m * stride + i], rows padded to whole groups of
+// 0-9 and the neighbour's type in bits 10-12, two per word, word m of the
+// row at nbr16[m * stride + column] (the column of the lane order below), rows padded to whole groups of
 // HALO_GROUP entries with the particle's own local index and type 0
 // (distance 0: no contribution).  Which ext slot an entry gets depends on the order of
 // the LDS atomics; the values staged at those slots, and so the forces, do
-// not.  The set stops taking entries once HALO_EXT distinct ones are in, so
-// at most HALO_EXT + BLOCK of its HALO_HASH slots fill and every probe
+// not.  The set stops taking entries once ext_cap (the halo capacity less
+// BLOCK) distinct ones are in, so at most HALO_EXT_MAX + BLOCK of its HALO_HASH slots fill and every probe
 // sequence ends at an empty or equal slot.
 constexpr int HALO_HASH = 4096;
 constexpr uint32_t HALO_EMPTY = 0xffffffffu;
-static_assert(HALO_EXT + BLOCK < HALO_HASH, "halo hash set too small");
-static_assert(HALO_CAP == BLOCK + HALO_EXT && HALO_CAP <= 1024, "10-bit halo indices");
+static_assert(HALO_EXT_MAX + BLOCK < HALO_HASH, "halo hash set too small");
+static_assert(HALO_CAP == BLOCK + HALO_EXT && HALO_CAP <= HALO_CAP_WIDE && HALO_CAP_WIDE <= 1024, "10-bit halo indices");
 static_assert(NT <= 8, "3-bit types in the halo entries");
 
 __device__ inline uint32_t halo_hash(uint32_t j) { return (j * 2654435761u) >> (32 - 12); }   // 12 = log2(HALO_HASH)
 
 __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, const uint32_t *nbr,
                                                 const uint32_t *nnbr, const uint8_t *type, uint32_t *ext,
-                                                uint32_t *ext_cnt, uint32_t *nbr16, uint8_t *hord) {
+                                                uint32_t *ext_cnt, uint32_t *nbr16, uint8_t *hord, int ext_cap) {
     __shared__ uint32_t s_key[HALO_HASH];
     __shared__ uint16_t s_val[HALO_HASH];
     __shared__ int s_nn[BLOCK];
@@ -640,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
     __syncthreads();
     const int nn = i < n ? (int)nnbr[i] : 0;
     const uint32_t self = (uint32_t)i;
-    uint32_t *ext_b = ext + b * HALO_EXT;
+    uint32_t *ext_b = ext + b * HALO_EXT_MAX;
     // entries in batches of HB (the loads of a batch in flight together);
     // own-block entries and the padding (the particle itself) are skipped
     constexpr int HB = 8;
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(BLOCK) void k_halo(long long n, long long stride, c
                 const uint32_t prev = atomicCAS(&s_key[h], HALO_EMPTY, j);
                 if (prev == HALO_EMPTY) {
                     const int e = atomicAdd(&s_cnt, 1);
-                    if (e < HALO_EXT) {
+                    if (e < ext_cap) {
                         s_val[h] = (uint16_t)e;
                         ext_b[e] = j;
                     } else {
@@ -871,9 +871,10 @@ void launch_budget(long long n, const double *x, const uint8_t *type, const uint
 }
 
 void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
-                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, uint8_t *hord, hipStream_t s) {
+                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, uint8_t *hord, int cap, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, type, ext, cnt, nbr16, hord);
+    hipLaunchKernelGGL(k_halo, dim3(nblk(n)), dim3(BLOCK), 0, s, n, stride, nbr, nnbr, type, ext, cnt, nbr16, hord,
+                       cap - BLOCK);
 }
 
 void launch_disp(long long n, const double *x, const uint32_t *anc, const float *rb, Anchor g, const float *bud2,

@@ -749,8 +749,9 @@ __device__ inline void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2, u
 // LDS of a block-halo step kernel: the halo's mirrors (own particles at
 // [0, BLOCK), outside neighbours after them), one record per type pair, the
 // bond constants.
+template <int CAP>
 struct HaloLDS {
-    float4 x[HALO_CAP], n[HALO_CAP];
+    float4 x[CAP], n[CAP];
     PairRec pp[36];
     float K[NBT], r0[NBT];
 };
@@ -758,7 +759,8 @@ struct HaloLDS {
 // Every thread of the block: the constants and, unless the block's halo
 // overflowed (*hcnt = HALO_OVERFLOW), the halo mirrors; returns the mirror of
 // particle i (blk: this block's particle block).
-__device__ __forceinline__ PQ halo_stage(const StepArgs &a, long long blk, bool act, long long i, HaloLDS &L,
+template <int CAP>
+__device__ __forceinline__ PQ halo_stage(const StepArgs &a, long long blk, bool act, long long i, HaloLDS<CAP> &L,
                                          uint32_t &hcnt) {
     load_pair_recs(L.pp);
     if (threadIdx.x < NBT) { L.K[threadIdx.x] = c_fff.bond_K[threadIdx.x]; L.r0[threadIdx.x] = c_fff.bond_r0[threadIdx.x]; }
@@ -767,7 +769,7 @@ __device__ __forceinline__ PQ halo_stage(const StepArgs &a, long long blk, bool 
     if (act) qi = a.pq_in[i];
     if (hcnt != HALO_OVERFLOW) {
         L.x[threadIdx.x] = qi.x; L.n[threadIdx.x] = qi.n;
-        const uint32_t *ext = a.halo_ext + blk * HALO_EXT;
+        const uint32_t *ext = a.halo_ext + blk * HALO_EXT_MAX;
         for (uint32_t q = threadIdx.x; q < hcnt; q += BLOCK) {
             const PQ p = a.pq_in[ext[q]];
             L.x[BLOCK + q] = p.x; L.n[BLOCK + q] = p.n;
@@ -780,8 +782,10 @@ __device__ __forceinline__ PQ halo_stage(const StepArgs &a, long long blk, bool 
 // pair and bond forces of active particle i from the staged halo (its row at
 // column `col` of the 16-bit list), or, for an overflowed block, gathered one
 // neighbour at a time
+template <int CAP>
 __device__ __forceinline__ void halo_accumulate(const StepArgs &a, const Anchor &ag, long long i, long long col,
-                                                const PQ &qi, uint32_t hcnt, const HaloLDS &L, Acc<double> &acc) {
+                                                const PQ &qi, uint32_t hcnt, const HaloLDS<CAP> &L,
+                                                Acc<double> &acc) {
     const int ti = __float_as_int(qi.n.w);
     const AnchorI ai = unpack(__float_as_uint(qi.x.w), ag);
     const int nn = (int)a.nnbr[i];
@@ -807,19 +811,21 @@ __device__ __forceinline__ void halo_accumulate(const StepArgs &a, const Anchor 
 struct alignas(16) XN64 {
     double x[3], n[3];
 };
+template <int CAP>
 struct HaloLDS64 {
-    XN64 r[HALO_CAP];
+    XN64 r[CAP];
 };
 
+template <int CAP>
 __device__ __forceinline__ void halo_stage64(const StepArgs &a, long long blk, bool act, long long i,
-                                             HaloLDS64 &L, uint32_t &hcnt) {
+                                             HaloLDS64<CAP> &L, uint32_t &hcnt) {
     hcnt = a.halo_cnt[blk];
     if (hcnt != HALO_OVERFLOW) {
         XN64 q{};
         if (act)
             for (int d = 0; d < 3; ++d) { q.x[d] = a.x_in[3 * i + d]; q.n[d] = a.n_in[3 * i + d]; }
         L.r[threadIdx.x] = q;
-        const uint32_t *ext = a.halo_ext + blk * HALO_EXT;
+        const uint32_t *ext = a.halo_ext + blk * HALO_EXT_MAX;
         for (uint32_t e = threadIdx.x; e < hcnt; e += BLOCK) {
             const long long j = ext[e];
             XN64 p;
@@ -830,8 +836,9 @@ __device__ __forceinline__ void halo_stage64(const StepArgs &a, long long blk, b
     __syncthreads();
 }
 
+template <int CAP>
 __device__ __forceinline__ void halo_accumulate64(const StepArgs &a, const Anchor &ag, long long i, long long col,
-                                                  uint32_t hcnt, const HaloLDS64 &L, const FFS<double> &t,
+                                                  uint32_t hcnt, const HaloLDS64<CAP> &L, const FFS<double> &t,
                                                   Acc<double> &acc) {
     const double xi = a.x_in[3 * i], yi = a.x_in[3 * i + 1], zi = a.x_in[3 * i + 2];
     const double nix = a.n_in[3 * i], niy = a.n_in[3 * i + 1], niz = a.n_in[3 * i + 2];
@@ -866,7 +873,7 @@ __device__ __forceinline__ void halo_accumulate64(const StepArgs &a, const Ancho
     }
 }
 
-template <bool MIXED, bool HALO>
+template <bool MIXED, bool HALO, int CAP = HALO_CAP>
 __device__ __forceinline__ void step_langevin(const StepArgs &a) {
     // the anchor grid from its device copy (rewritten at every neighbour
     // rebuild, so a captured batch stays valid), a uniform load at the top
@@ -885,8 +892,8 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
     // force-field constants: the FP32/FP64 tables, or the block halo with
     // its constants (each instantiation keeps only the LDS it references)
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
-    __shared__ HaloLDS hl;
-    __shared__ HaloLDS64 hl64;
+    __shared__ HaloLDS<CAP> hl;
+    __shared__ HaloLDS64<CAP> hl64;
     const long long blk = block_index(a.swizzle);
     // this lane's position in the block (its column of the 16-bit list),
     // recomputed where needed rather than kept live
@@ -1010,6 +1017,14 @@ __global__ __launch_bounds__(BLOCK) ORBC_HALO_ATTR void k_step_langevin_halo(Ste
 }
 // FP64 halo: 48-byte records, 3 blocks of 256 per CU's LDS
 __global__ __launch_bounds__(BLOCK) void k_step_langevin64_halo(StepArgs a) { step_langevin<false, true>(a); }
+// wide halos (HALO_CAP_WIDE local indices): the mixed kernel at 4 blocks of
+// 256 per CU (128 registers), the FP64 one still at 3
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_step_langevin_halo_wide(StepArgs a) {
+    step_langevin<true, true, HALO_CAP_WIDE>(a);
+}
+__global__ __launch_bounds__(BLOCK) void k_step_langevin64_halo_wide(StepArgs a) {
+    step_langevin<false, true, HALO_CAP_WIDE>(a);
+}
 
 // ---- fused forces + post_toque_final_update (NH second half) --------------
 // src/integrate_nh.h:237-273 after compute_pairwise_fused/compute_bonded, in
@@ -1017,7 +1032,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_langevin64_halo(StepArgs a) { st
 // are rounded to FP32 as the FP32 force arrays hold them; t = n x t;
 // v += dt/2 (f/m - zeta v); o += dt/2 t; f and n x t stored (FP32 arrays in
 // mixed precision) for the next first half; KE partial per particle block.
-template <bool MIXED, bool HALO>
+template <bool MIXED, bool HALO, int CAP = HALO_CAP>
 __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, float *t32, double *f64, double *t64,
                                                double *ke_partial, const double *zeta_p) {
     if (a.nh_gate) {
@@ -1031,8 +1046,8 @@ __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, fl
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.done, 1);
     }
     __shared__ FFS<typename std::conditional<MIXED, float, double>::type> tab;
-    __shared__ HaloLDS hl;
-    __shared__ HaloLDS64 hl64;
+    __shared__ HaloLDS<CAP> hl;
+    __shared__ HaloLDS64<CAP> hl64;
     const long long blk = block_index(a.swizzle);
     uint32_t hcnt = HALO_OVERFLOW;
     PQ qh{};
@@ -1144,6 +1159,15 @@ __global__ __launch_bounds__(BLOCK) void k_step_nh_second64_halo(StepArgs a, flo
                                                                  const double *zeta_p) {
     step_nh_second<false, true>(a, f32, t32, f64, t64, ke_partial, zeta_p);
 }
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_step_nh_second_halo_wide(
+    StepArgs a, float *f32, float *t32, double *f64, double *t64, double *ke_partial, const double *zeta_p) {
+    step_nh_second<true, true, HALO_CAP_WIDE>(a, f32, t32, f64, t64, ke_partial, zeta_p);
+}
+__global__ __launch_bounds__(BLOCK) void k_step_nh_second64_halo_wide(StepArgs a, float *f32, float *t32,
+                                                                      double *f64, double *t64, double *ke_partial,
+                                                                      const double *zeta_p) {
+    step_nh_second<false, true, HALO_CAP_WIDE>(a, f32, t32, f64, t64, ke_partial, zeta_p);
+}
 
 inline unsigned nblk(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
@@ -1173,8 +1197,11 @@ void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool 
 void launch_step_langevin(const StepArgs &a, bool mixed, hipStream_t s) {
     if (a.n <= 0) return;
     const dim3 g(nblk(a.n)), b(BLOCK);
-    if (mixed && a.halo_cnt) hipLaunchKernelGGL(k_step_langevin_halo, g, b, 0, s, a);
+    const bool wide = a.halo_cap == HALO_CAP_WIDE;
+    if (mixed && a.halo_cnt && wide) hipLaunchKernelGGL(k_step_langevin_halo_wide, g, b, 0, s, a);
+    else if (mixed && a.halo_cnt) hipLaunchKernelGGL(k_step_langevin_halo, g, b, 0, s, a);
     else if (mixed) hipLaunchKernelGGL(k_step_langevin<true>, g, b, 0, s, a);
+    else if (a.halo_cnt && wide) hipLaunchKernelGGL(k_step_langevin64_halo_wide, g, b, 0, s, a);
     else if (a.halo_cnt) hipLaunchKernelGGL(k_step_langevin64_halo, g, b, 0, s, a);
     else hipLaunchKernelGGL(k_step_langevin<false>, g, b, 0, s, a);
 }
@@ -1183,9 +1210,14 @@ void launch_step_nh_second(const StepArgs &a, bool mixed, float *f32, float *t32
                            double *ke_partial, const double *zeta, hipStream_t s) {
     if (a.n <= 0) return;
     const dim3 g(nblk(a.n)), b(BLOCK);
-    if (mixed && a.halo_cnt)
+    const bool wide = a.halo_cap == HALO_CAP_WIDE;
+    if (mixed && a.halo_cnt && wide)
+        hipLaunchKernelGGL(k_step_nh_second_halo_wide, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
+    else if (mixed && a.halo_cnt)
         hipLaunchKernelGGL(k_step_nh_second_halo, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
     else if (mixed) hipLaunchKernelGGL(k_step_nh_second<true>, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
+    else if (a.halo_cnt && wide)
+        hipLaunchKernelGGL(k_step_nh_second64_halo_wide, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
     else if (a.halo_cnt)
         hipLaunchKernelGGL(k_step_nh_second64_halo, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);
     else hipLaunchKernelGGL(k_step_nh_second<false>, g, b, 0, s, a, f32, t32, f64, t64, ke_partial, zeta);

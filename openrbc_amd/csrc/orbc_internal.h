@@ -144,8 +144,12 @@ struct PQ {
 #ifndef ORBC_HALO_CAP
 #define ORBC_HALO_CAP 896
 #endif
-constexpr int HALO_CAP = ORBC_HALO_CAP;        // local indices: BLOCK own + HALO_EXT outside
+constexpr int HALO_CAP = ORBC_HALO_CAP;        // local indices: BLOCK own + HALO_EXT outside (default)
 constexpr int HALO_EXT = HALO_CAP - BLOCK;
+// the wide variant (the autotuner's halo capacity knob): 1024 local indices,
+// the most the 10-bit index holds, at one block fewer per CU
+constexpr int HALO_CAP_WIDE = 1024;
+constexpr int HALO_EXT_MAX = HALO_CAP_WIDE - BLOCK;   // ext entries kept per block (array stride)
 constexpr uint32_t HALO_OVERFLOW = 0xffffffffu;
 // entries per group of the LDS walk (rows are padded to whole groups)
 #ifndef ORBC_HALO_GROUP
@@ -175,6 +179,7 @@ struct StepArgs {
     // block halos (mixed path; null: every block gathers from nbr)
     const uint32_t *halo_ext, *halo_cnt, *nbr16;
     const uint8_t *hord;     // lane order of each halo block (k_halo): lane r takes particle blk * 256 + hord[.. + r]
+    int halo_cap;            // local indices of the halos (HALO_CAP or HALO_CAP_WIDE): the step-kernel variant
     const uint32_t *bnd;
     const uint8_t *nbnd;
     int has_bonds;
@@ -316,7 +321,7 @@ void launch_bondlist(long long n, long long stride, const uint32_t *slot, const 
 // block halos from the 32-bit list: ext (HALO_EXT per block), cnt (per block),
 // nbr16 (halo_words(kmax) words per particle, column stride `stride`)
 void launch_halo(long long n, long long stride, const uint32_t *nbr, const uint32_t *nnbr, const uint8_t *type,
-                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, uint8_t *hord, hipStream_t s);
+                 uint32_t *ext, uint32_t *cnt, uint32_t *nbr16, uint8_t *hord, int cap, hipStream_t s);
 
 // k_force.hip
 void launch_forces(const ForceArgs &a, bool mixed, bool energy, bool pair, bool bond, hipStream_t s);

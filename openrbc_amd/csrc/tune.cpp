@@ -91,15 +91,19 @@ struct Knobs {
     int block_halo;
     double cg_edge;
     int batch;
+    int halo_cap;
 };
 
-Knobs current(const orbc_ctx *c) { return Knobs{c->cfg.skin, c->cfg.block_halo, c->cg_edge, c->batch}; }
+Knobs current(const orbc_ctx *c) {
+    return Knobs{c->cfg.skin, c->cfg.block_halo, c->cg_edge, c->batch, c->halo_cap};
+}
 
 void apply(orbc_ctx *c, const Knobs &k) {
     c->cfg.skin = k.skin;
     c->cfg.block_halo = k.block_halo;
     c->cg_edge = k.cg_edge;
     c->batch = k.batch;
+    c->halo_cap = k.halo_cap;
     c->need_rebuild = true;
 }
 
@@ -147,6 +151,10 @@ int orbc_autotune(orbc_ctx *c, int warm_steps, int trial_steps, const char *tabl
             };
             sweep([](Knobs &k, double v) { k.skin = v; }, {0.4, 0.5, 0.6, 0.7, 0.8});
             sweep([](Knobs &k, double v) { k.block_halo = (int)v; }, {0, 1, 2});
+            // the tile shape of the halo kernels: 896 local indices at 5 blocks
+            // per CU, or 1024 at 4 (fewer blocks overflow into global gathers)
+            if (best.block_halo != 0)
+                sweep([](Knobs &k, double v) { k.halo_cap = (int)v; }, {(double)HALO_CAP, (double)HALO_CAP_WIDE});
             if (cells) sweep([](Knobs &k, double v) { k.cg_edge = v; }, {6.0, 8.0, 10.0});
             sweep([](Knobs &k, double v) { k.batch = (int)v; }, {8, 16, 32});
             c->tuning = false;
@@ -156,6 +164,7 @@ int orbc_autotune(orbc_ctx *c, int warm_steps, int trial_steps, const char *tabl
                 out->skin = best.skin; out->block_halo = best.block_halo; out->centroid_cell = best.cg_edge;
                 out->batch = best.batch; out->ms_per_step = best_ms; out->ms_per_step_start = t_default;
                 out->trials = (int)trials.size();
+                out->halo_capacity = best.halo_cap;
             }
         } catch (...) {
             c->tuning = false;
@@ -175,12 +184,12 @@ int orbc_autotune(orbc_ctx *c, int warm_steps, int trial_steps, const char *tabl
                          trial_steps);
             for (size_t k = 0; k < trials.size(); ++k)
                 std::fprintf(fp, "%s{\"skin\": %g, \"block_halo\": %d, \"centroid_cell\": %g, \"batch\": %d, "
-                                 "\"ms_per_step\": %.5f}",
+                                 "\"halo_capacity\": %d, \"ms_per_step\": %.5f}",
                              k ? ", " : "", trials[k].k.skin, trials[k].k.block_halo, trials[k].k.cg_edge,
-                             trials[k].k.batch, trials[k].ms);
+                             trials[k].k.batch, trials[k].k.halo_cap, trials[k].ms);
             std::fprintf(fp, "], \"best\": {\"skin\": %g, \"block_halo\": %d, \"centroid_cell\": %g, \"batch\": %d, "
-                             "\"ms_per_step\": %.5f}}\n",
-                         best.skin, best.block_halo, best.cg_edge, best.batch, best_ms);
+                             "\"halo_capacity\": %d, \"ms_per_step\": %.5f}}\n",
+                         best.skin, best.block_halo, best.cg_edge, best.batch, best.halo_cap, best_ms);
             std::fclose(fp);
         }
         return (int)ORBC_OK;
@@ -190,9 +199,11 @@ int orbc_autotune(orbc_ctx *c, int warm_steps, int trial_steps, const char *tabl
 int orbc_set_tuning(orbc_ctx *c, const orbc_tuning *t) {
     return guarded(c, [&] {
         if (!t || !(t->skin > 0) || t->block_halo < 0 || t->block_halo > 2 || !(t->centroid_cell > 0) ||
-            t->batch < 1 || t->batch > 32)
+            t->batch < 1 || t->batch > 32 ||
+            (t->halo_capacity != 0 && t->halo_capacity != HALO_CAP && t->halo_capacity != HALO_CAP_WIDE))
             return fail(c, ORBC_ERR_INVALID, "bad tuning");
-        apply(c, Knobs{t->skin, t->block_halo, t->centroid_cell, t->batch});
+        apply(c, Knobs{t->skin, t->block_halo, t->centroid_cell, t->batch,
+                       t->halo_capacity ? t->halo_capacity : c->halo_cap});
         return (int)ORBC_OK;
     });
 }
@@ -201,6 +212,7 @@ int orbc_get_tuning(const orbc_ctx *c, orbc_tuning *t) {
     if (!c || !t) return ORBC_ERR_INVALID;
     std::memset(t, 0, sizeof(*t));
     t->skin = c->cfg.skin; t->block_halo = c->cfg.block_halo; t->centroid_cell = c->cg_edge; t->batch = c->batch;
+    t->halo_capacity = c->halo_cap;
     return ORBC_OK;
 }
 

@@ -52,9 +52,11 @@ def _with_blob(s, m=16, spacing=1.0, offset=(400.0, 0.0, 0.0), seed=3):
     return out
 
 
-def _run(s, halo, nsteps, minimize=0, cells=False, mode=None, precision=orbc.MIXED, **kw):
+def _run(s, halo, nsteps, minimize=0, cells=False, mode=None, precision=orbc.MIXED, halo_capacity=0, **kw):
     # block_halo 2: halos at every rebuild; 0: never; 1: while lists last 4+ steps
     sim = orbc.Simulation(precision=precision, block_halo=2 * halo if mode is None else mode, **kw)
+    if halo_capacity:
+        sim.set_tuning(**{**sim.tuning(), "halo_capacity": halo_capacity})
     t = s["type"]
     lip = np.where(t == 0)[0]
     prot = np.where(t != 0)[0]
@@ -114,6 +116,21 @@ def test_block_halo_overflow_blocks_gather_globally(rbc, thermostat, precision):
     off, _ = _run(s, 0, 6, **kw)
     _same(on, off)
     assert 0 < obs_on["halo_overflow"] < obs_on["halo_blocks"]
+
+
+@pytest.mark.parametrize("precision", [orbc.MIXED, orbc.FP64])
+@pytest.mark.parametrize("thermostat", [orbc.RUN_LANGEVIN, orbc.RUN_NOSE_HOOVER])
+def test_wide_halo_kernels_bit_identical(rbc, thermostat, precision):
+    """The wide halo kernels (1024 local indices, the autotuner's halo
+    capacity knob) against gathers, with blocks that overflow even them."""
+    s = _with_blob(rbc)
+    print(f"wide halo run: n={len(s['type'])}, thermostat {thermostat}, precision {precision}", flush=True)
+    kw = dict(minimize=20, seed=4, skin=0.2, thermostat=thermostat, precision=precision)
+    wide, obs_w = _run(s, 1, 6, halo_capacity=1024, **kw)
+    off, _ = _run(s, 0, 6, **kw)
+    _, obs_d = _run(s, 1, 6, **kw)
+    _same(wide, off)
+    assert 0 < obs_w["halo_overflow"] <= obs_d["halo_overflow"] < obs_w["halo_blocks"]
 
 
 @pytest.mark.parametrize("halo", [0, 1])
