@@ -399,7 +399,18 @@ def test_voronoi_stencil_pair_set(system, precision, oracle):
         assert len(got) > 0 and np.array_equal(got, exp)
         full = id_pairs(s, oracle.pairs_brute(s["x"], s["type"]))
     else:
-        full = id_pairs(s, oracle.pairs_mixed(s["x"], s["type"], sim.observables()["anchor_h"]))
+        pm = oracle.pairs_mixed(s["x"], s["type"], sim.observables()["anchor_h"])
+        full = id_pairs(s, pm)
+        # the stencil test of the neighbour build in FP32: the squared
+        # distance of the two cells' FP32 centroids, summed unfused, below
+        # 6^2 / 8^2 / 9^2 for lipid-lipid / one lipid / no lipid
+        c32 = np.asarray(c, np.float32)
+        d = c32[aff[pm[:, 1]]] - c32[aff[pm[:, 0]]]
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        nl = (s["type"][pm[:, 0]] == 0).astype(int) + (s["type"][pm[:, 1]] == 0)
+        R = np.where(nl == 2, 6.0, np.where(nl == 1, 8.0, 9.0)).astype(np.float32)
+        exp = id_pairs(s, pm[d2 < R * R])
+        assert len(got) > 0 and np.array_equal(got, exp)
     # the stencil keeps only in-cutoff pairs, and may drop some of them
     fs = set(map(tuple, full.tolist()))
     assert 0 < len(got) <= len(full) and all(tuple(p) in fs for p in got.tolist())
@@ -602,6 +613,9 @@ def test_stray_cleanup_vs_oracle(precision, oracle):
     sphere, lifted = _lifted_sphere()
     sim = orbc.Simulation(precision=precision, seed=5, freq_cleanup=0)
     orbc.system_to_simulation(sphere, sim)
+    # relax the random sphere's near-overlapping lipids (the lifted ones have
+    # no neighbours and stay where they are)
+    sim.minimize(200)
     sim.voronoi_init(0, 4)
     c, aff, _ = sim.voronoi_get()
     before = sim.get_state(orbc.LIPID)
@@ -625,12 +639,7 @@ def test_stray_cleanup_vs_oracle(precision, oracle):
     if precision == orbc.FP64:
         assert eps_f(f, of) < EPS_FP64 and eps_f(t, ot) < EPS_FP64
     else:
-        # an unrelaxed random sphere (near-overlapping lipids whose pair terms reach thousands
-        # while the sums are tens): FP32 mirror rounding gives up to ~5e-5 here, so this only
-        # checks the compacted store (a wrong neighbour or a stale particle is an O(1) error);
-        # the mixed-precision bound itself is tested on relaxed systems
-        fn = np.linalg.norm(of, axis=1)
-        assert np.max(np.linalg.norm(f - of, axis=1) / np.maximum(fn, np.median(fn))) < 1e-3
+        assert eps_f(f, of) < EPS_MIXED and eps_f(t, ot) < EPS_MIXED, (eps_f(f, of), eps_f(t, ot))
     # nothing left to remove with the cells recomputed
     sim.voronoi_update()
     assert sim.delete_stray_lipids(2.5) >= 0
