@@ -908,14 +908,14 @@ __device__ __forceinline__ void step_langevin(const StepArgs &a) {
     }
     // the particle of this lane: the block's lane order (k_halo) when its
     // neighbours are staged, else the lane's own position
-    long long i = ORBC_LANE_COL;
+    int l = (int)threadIdx.x;
     if constexpr (HALO) {
         if (hcnt != HALO_OVERFLOW) {
-            const int l = a.hord[ORBC_LANE_COL];
-            i = blk * BLOCK + l;
+            l = a.hord[ORBC_LANE_COL];
             if constexpr (MIXED) { qi.x = hl.x[l]; qi.n = hl.n[l]; }
         }
     }
+    const long long i = blk * BLOCK + l;
     const bool act = i < a.n;
     float d2 = 0.f;
     bool bad = false;
@@ -1058,14 +1058,14 @@ __device__ __forceinline__ void step_nh_second(const StepArgs &a, float *f32, fl
         if constexpr (HALO) halo_stage64(a, blk, ORBC_LANE_COL < a.n, ORBC_LANE_COL, hl64, hcnt);
     }
     // the particle of this lane (see step_langevin)
-    long long i = ORBC_LANE_COL;
+    int l = (int)threadIdx.x;
     if constexpr (HALO) {
         if (hcnt != HALO_OVERFLOW) {
-            const int l = a.hord[ORBC_LANE_COL];
-            i = blk * BLOCK + l;
+            l = a.hord[ORBC_LANE_COL];
             if constexpr (MIXED) { qh.x = hl.x[l]; qh.n = hl.n[l]; }
         }
     }
+    const long long i = blk * BLOCK + l;
     const bool act = i < a.n;
     double ke = 0.0;
     if (act) {
