@@ -288,19 +288,33 @@ __global__ __launch_bounds__(BLOCK) void k_sum_partials(const double *partial, l
 
 // zeta += dt/2 / Q (KE - 3/2 N kT), the functor destructors
 // src/integrate_nh.h:182-185,241-244
-__global__ __launch_bounds__(BLOCK) void k_zeta_update(const double *partial, long long nb, double *zeta, double dt,
-                                                       double Q, double nkT15, const int *stop) {
+// One workgroup of ZETA_THREADS sums the kinetic-energy partials in a fixed
+// order: four independent running sums per thread (so the loads of a thread
+// are in flight together), then the wave and workgroup trees.
+constexpr int ZETA_THREADS = 1024;
+__global__ __launch_bounds__(ZETA_THREADS) void k_zeta_update(const double *partial, long long nb, double *zeta,
+                                                              double dt, double Q, double nkT15, const int *stop) {
     if (stop && *stop) return;
-    double s = 0.0;
-    for (long long k = threadIdx.x; k < nb; k += BLOCK) s += partial[k];
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    long long k = threadIdx.x;
+    for (; k + 3 * ZETA_THREADS < nb; k += 4 * ZETA_THREADS) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] += partial[k + u * ZETA_THREADS];
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const long long q = k + (long long)u * ZETA_THREADS;
+        if (q < nb) s4[u] += partial[q];
+    }
+    double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    __shared__ double ws[BLOCK / 64];
+    __shared__ double ws[ZETA_THREADS / 64];
     if (__lane_id() == 0) ws[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         double ke = 0;
-        for (int w = 0; w < BLOCK / 64; ++w) ke += ws[w];
+        for (int w = 0; w < ZETA_THREADS / 64; ++w) ke += ws[w];
         *zeta += 0.5 * dt / Q * (ke - nkT15);
     }
 }
@@ -399,7 +413,7 @@ void launch_sum_partials(const double *partial, long long nb, double *out, hipSt
 
 void launch_zeta_update(const double *partial, long long nb, double *zeta, double dt, double Q, double nkT15,
                         hipStream_t s, const int *stop) {
-    hipLaunchKernelGGL(k_zeta_update, dim3(1), dim3(BLOCK), 0, s, partial, nb, zeta, dt, Q, nkT15, stop);
+    hipLaunchKernelGGL(k_zeta_update, dim3(1), dim3(ZETA_THREADS), 0, s, partial, nb, zeta, dt, Q, nkT15, stop);
 }
 
 void launch_sum_array(const double *a, long long n, double *partial, hipStream_t s) {
