@@ -364,7 +364,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+              "(how the path would shard over GPUs is described with the design notes).", file=sys.stderr)
         sys.exit(2)
 
     if args.workload != "rbc":

@@ -15,7 +15,7 @@ from _systems import patch, periodic_patch
 
 pytestmark = pytest.mark.gpu
 
-EPS_MIXED = 1e-5      # eps_F target for FP32 forces (BASELINE force-error definition)
+EPS_MIXED = 1e-5      # eps_F target of the mixed path: max |dF| / rms |F| against FP64
 EPS_FP64 = 1e-11
 
 
